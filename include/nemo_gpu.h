@@ -1,0 +1,146 @@
+/*
+ * nemo_gpu.h -- C-ABI of the MI355X (gfx950) spatial operator and RK stage
+ * update for nemo.
+ *
+ * The reference has no device boundary: its Fortran game loop calls the
+ * kernel/equations/timedisc modules directly (SURVEY of the reference:
+ * driver_mod.f90:62-150 -> hook_evolve_500 -> rkstep_cb -> kernel_evolve).
+ * This header is the thin layer a Fortran host binds with ISO_C_BINDING
+ * (nemo_amd/fortran/nemo_gpu_mod.f90), in the style of the reference's own
+ * C-ABI bindings (mesh/p4est/wrapper/p4wrap_interfaces_mod.f90:46-120):
+ * plain pointers, sizes and an opaque handle, `int` status returns.
+ *
+ * Conventions
+ *   - Every entry point returns NEMO_OK (0) or a positive error code; the
+ *     message of the last error on the calling thread is available from
+ *     nemo_gpu_error_message().  This replaces raise_error / rt%stat
+ *     (utils/status_mod.f90:7-21, runtime_mod.f90:55-61).
+ *   - Host state arrays use the reference's per-quad payload layout stacked
+ *     over elements: Fortran state(N_NODES,N_NODES,N_NODES,N_VARS,nelems),
+ *     i.e. C u[elem][var][node] with node = i + M*j + M*M*k, M = N+1
+ *     (kernel types: kernel/mcfv/3d/kernel_types_mod.f90:6-21).
+ *   - Face ids: 0..5 = NOR, SOU, WES, EAS, FRO, BAC of utils/constants_mod.f90:
+ *     34-40 (x-, x+, y-, y+, z-, z+).
+ *   - One host thread per handle; all device work is ordered on the
+ *     handle's HIP stream.
+ */
+#ifndef NEMO_GPU_H
+#define NEMO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEMO_OK 0
+#define NEMO_ERR_ARG 1         /* invalid argument or unsupported combination */
+#define NEMO_ERR_HIP 2         /* HIP runtime error */
+#define NEMO_ERR_STATE 3       /* positivity repair failed (kernel_repair false) */
+#define NEMO_ERR_NOT_READY 4   /* mesh or state not uploaded yet */
+
+/* equations (preproc.py selection of source/equations/...) */
+#define NEMO_EQ_EULER 0
+#define NEMO_EQ_GLM_MHD 1
+/* volume integral */
+#define NEMO_VOL_WEAK 0        /* kernel/dgsem/weakform (D-hat f) */
+#define NEMO_VOL_CENTRAL 1     /* flux differencing, two_point/standard */
+#define NEMO_VOL_EC 2          /* flux differencing, two_point/chandrashekar */
+/* surface Riemann solver (riemann_mod slots, scripts/nemo.py:398-415) */
+#define NEMO_SURF_RUSANOV 0    /* equations/euler/riemann/rusanov */
+#define NEMO_SURF_HLLC 1       /* equations/euler/riemann/hllc (HLLC/HLLE hybrid) */
+/* troubled-cell indicator for DG/FV blending */
+#define NEMO_IND_NONE 0        /* pure DG */
+#define NEMO_IND_PP 1          /* Persson-Peraire on rho*p */
+/* time integration (timedisc/rungekutta/...) */
+#define NEMO_TD_LSRK54 0       /* low-storage/4-5 */
+#define NEMO_TD_SSPRK54 1      /* ssp/4-5 */
+
+/* Compile-time configuration of the reference (preproc.py flags and
+ * setup_config_mod constants) as a runtime struct.  Fortran: a bind(c)
+ * derived type with the same members. */
+typedef struct nemo_gpu_config {
+    int32_t n_poly;        /* N; N_NODES = N+1 (PP_N_NODES), 2..4 */
+    int32_t equations;     /* NEMO_EQ_* */
+    int32_t volume;        /* NEMO_VOL_* */
+    int32_t surface;       /* NEMO_SURF_* */
+    int32_t indicator;     /* NEMO_IND_* */
+    int32_t timedisc;      /* NEMO_TD_* */
+    int32_t repair;        /* 1: kernel_repair after every stage */
+    int32_t precision;     /* 0: FP64 operator */
+    double gamma;          /* setup_config_mod::kappa */
+    double alpha_max;      /* blending factor clamp */
+    double alpha_min;      /* blending factors below this become 0 */
+    double cfl;            /* setup_config_mod::CFL */
+    double mu0;            /* MHD only */
+} nemo_gpu_config;
+
+/* Create / destroy a solver (replaces hook_init_400 operator setup,
+ * kernel/dgsem/weakform/2d/kernel_share_mod.f90:56-151). */
+int nemo_gpu_create(const nemo_gpu_config *cfg, void **handle);
+int nemo_gpu_destroy(void *handle);
+
+/* Upload mesh connectivity once per remesh (replaces the quad/side tables
+ * of mesh_alloc, mesh/p4est/mesh_utils_mod.f90:160-441): nbr[nelem][6]
+ * element across each face, delta[nelem][3] element edge lengths
+ * (quad%delta).  Conforming, aligned faces. */
+int nemo_gpu_upload_mesh(void *handle, int64_t nelem, const int32_t *nbr, const double *delta);
+
+/* State upload/download (payload quad%pld%hydro%state of every quad). */
+int nemo_gpu_upload_state(void *handle, const double *u);
+int nemo_gpu_download_state(void *handle, double *u);
+
+/* RHS of the current state, du/dt per node (replaces kernel_calc_flux,
+ * kernel/dgsem/weakform/2d/kernel_flux_mod.f90:197-277, after the face pass
+ * hook_pre_rk_timestep_300).  Parity entry point. */
+int nemo_gpu_rhs(void *handle, double *rhs);
+
+/* Blending factor per element for the current state (indicator only). */
+int nemo_gpu_indicator(void *handle, double *alpha);
+
+/* CFL time step of the current state (replaces hook_calc_timestep_200,
+ * kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-63): dt =
+ * 0.5*CFL/(2*N_NODES-1)*min(delta/lambda); lmax and vmax are the global
+ * maxima used for GLM cleaning.  Synchronises. */
+int nemo_gpu_calc_timestep(void *handle, double *dt, double *lmax, double *vmax);
+
+/* One RK stage s = 1..5 with time step dt (replaces hook_mid_rk_timestep_500
+ * -> rkstep_cb -> kernel_evolve + timedisc_probe + kernel_repair,
+ * timedisc/rungekutta/low-storage/4-5/timedisc_hooks_mod.f90:19-56). */
+int nemo_gpu_stage(void *handle, int32_t stage, double dt);
+
+/* All stages of one time step (hook_evolve_500,
+ * timedisc/rungekutta/timedisc_evolve_mod.f90:20-84), asynchronous. */
+int nemo_gpu_step(void *handle, double dt);
+
+/* nsteps steps with the CFL time step recomputed on the device before each
+ * step (driver_mod.f90:62-150 without checkpoint/analysis), no host
+ * synchronisation between steps.  dt_hist (may be NULL) receives the time
+ * step of every step; this call synchronises. */
+int nemo_gpu_run(void *handle, int32_t nsteps, double *dt_hist);
+
+/* Wait for all queued work; returns NEMO_ERR_STATE if a repair failed
+ * since the last call (rt%stat%ok check, timedisc_evolve_mod.f90:68-79). */
+int nemo_gpu_synchronize(void *handle);
+int nemo_gpu_get_status(void *handle, int32_t *code);
+
+/* Kernel timing with HIP events on the solver's stream: enable, then read
+ * the number of stage launches and their summed duration in milliseconds
+ * (reading synchronises and resets the counters). */
+int nemo_gpu_set_timing(void *handle, int32_t enable);
+int nemo_gpu_read_timing(void *handle, int64_t *launches, double *total_ms);
+
+/* Operators for polynomial degree N (host only, no device needed):
+ * LGL nodes x[M], weights w[M] (utils/interpol_mod.f90:513-582),
+ * D[M][M] with D[i][j] = l_j'(x_i) (interpol_mod.f90:38-80), and the
+ * inverse normalised-Legendre Vandermonde Vinv[M][M] (interpol_mod.f90:
+ * 584-601). */
+int nemo_gpu_operators(int32_t n_poly, double *x, double *w, double *D, double *Vinv);
+
+/* Last error message on this thread (NUL-terminated, truncated to len). */
+int nemo_gpu_error_message(char *buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""Loader for the in-tree HIP library nemo_amd/libnemo_gpu.so (C-ABI of
+include/nemo_gpu.h).  There is no fallback: if the library is missing the
+import fails loudly."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnemo_gpu.so")
+
+NEMO_OK = 0
+NEMO_ERR_ARG = 1
+NEMO_ERR_HIP = 2
+NEMO_ERR_STATE = 3
+NEMO_ERR_NOT_READY = 4
+
+EQ_EULER, EQ_GLM_MHD = 0, 1
+VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
+SURF_RUSANOV, SURF_HLLC = 0, 1
+IND_NONE, IND_PP = 0, 1
+TD_LSRK54, TD_SSPRK54 = 0, 1
+
+
+class Config(ctypes.Structure):
+    """nemo_gpu_config of include/nemo_gpu.h."""
+    _fields_ = [
+        ("n_poly", ctypes.c_int32),
+        ("equations", ctypes.c_int32),
+        ("volume", ctypes.c_int32),
+        ("surface", ctypes.c_int32),
+        ("indicator", ctypes.c_int32),
+        ("timedisc", ctypes.c_int32),
+        ("repair", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("alpha_max", ctypes.c_double),
+        ("alpha_min", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("mu0", ctypes.c_double),
+    ]
+
+
+class NemoGpuError(RuntimeError):
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where} failed with code {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+_P = ctypes.POINTER(ctypes.c_double)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+SIGNATURES = {
+    "nemo_gpu_create": [ctypes.POINTER(Config), ctypes.POINTER(_VP)],
+    "nemo_gpu_destroy": [_VP],
+    "nemo_gpu_upload_mesh": [_VP, ctypes.c_int64, _PI32, _P],
+    "nemo_gpu_upload_state": [_VP, _P],
+    "nemo_gpu_download_state": [_VP, _P],
+    "nemo_gpu_rhs": [_VP, _P],
+    "nemo_gpu_indicator": [_VP, _P],
+    "nemo_gpu_calc_timestep": [_VP, _P, _P, _P],
+    "nemo_gpu_stage": [_VP, ctypes.c_int32, ctypes.c_double],
+    "nemo_gpu_step": [_VP, ctypes.c_double],
+    "nemo_gpu_run": [_VP, ctypes.c_int32, _P],
+    "nemo_gpu_synchronize": [_VP],
+    "nemo_gpu_get_status": [_VP, _PI32],
+    "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
+    "nemo_gpu_read_timing": [_VP, _PI64, _P],
+    "nemo_gpu_operators": [ctypes.c_int32, _P, _P, _P, _P],
+    "nemo_gpu_error_message": [ctypes.c_char_p, ctypes.c_int32],
+}
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"nemo_amd: HIP library {LIB_PATH} is not built; run __graft_entry__.build() "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().nemo_gpu_error_message(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, where: str) -> None:
+    if rc != NEMO_OK:
+        raise NemoGpuError(rc, where, last_error())
+
+
+def dptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_PI32)

@@ -1,0 +1,61 @@
+"""Build recipes for the in-tree native code (gfx950 HIP library and the
+Fortran host driver)."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+FC = os.path.join(ROCM, "bin", "amdflang")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions stay disabled (build flags)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=ROOT):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "not a recognized feature" not in l
+                    and "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_hip(verbose: bool = False) -> str:
+    src = os.path.join(HERE, "csrc", "nemo_gpu.hip")
+    out = os.path.join(HERE, "libnemo_gpu.so")
+    tmp = out + ".tmp"
+    log = _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    os.replace(tmp, out)
+    if verbose and log:
+        print(log)
+    return out
+
+
+def build_fortran_driver() -> str:
+    """Fortran host driver (nemo_amd/fortran) linked against libnemo_gpu.so."""
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(fdir, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(HERE, "nemo_gpu_driver")
+    srcs = ["nemo_gpu_mod.f90", "nemo_setup_mod.f90", "nemo_driver.f90"]
+    for s in srcs:
+        _run([FC, "-O2", "-J", bdir, "-c", os.path.join(fdir, s), "-o",
+              os.path.join(bdir, s.replace(".f90", ".o"))])
+    objs = [os.path.join(bdir, s.replace(".f90", ".o")) for s in srcs]
+    _run([FC, "-o", exe, *objs, "-L" + HERE, "-lnemo_gpu", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    return exe
+
+
+def build_oracle() -> None:
+    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+    if os.path.isdir("/root/reference/source"):
+        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])

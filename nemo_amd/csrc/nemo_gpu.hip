@@ -1,0 +1,686 @@
+// nemo_gpu.hip -- C-ABI implementation (include/nemo_gpu.h) for gfx950.
+//
+// Host side of the MI355X spatial operator: operator construction, device
+// memory, the per-stage launch sequence and the error convention.  The device
+// work lives in dg_stage.hpp.  There is no CPU fallback: every compute entry
+// point runs HIP kernels and reports HIP errors.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nemo_gpu.h"
+#include "dg_stage.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return fail(NEMO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Operators on the host (own implementation of the reference's algorithms)
+// ---------------------------------------------------------------------------
+
+constexpr int kMaxM = 8;
+
+// Legendre recursion used by the LGL Newton iteration
+// (qAndLEvaluation, utils/interpol_mod.f90:468-509).
+void q_and_L(int N, double x, double& q, double& dq, double& LN) {
+    double Lm2 = 1.0, Lm1 = x, dLm2 = 0.0, dLm1 = 1.0, L = 0.0, dL = 0.0;
+    for (int k = 2; k <= N; ++k) {
+        const double rk = double(k);
+        L = (2.0 * rk - 1.0) / rk * x * Lm1 - (rk - 1.0) / rk * Lm2;
+        dL = dLm2 + (2.0 * rk - 1.0) * Lm1;
+        Lm2 = Lm1;
+        Lm1 = L;
+        dLm2 = dLm1;
+        dLm1 = dL;
+    }
+    const double rk = double(N + 1);
+    const double Lk = (2.0 * rk - 1.0) / rk * x * L - (rk - 1.0) / rk * Lm2;
+    const double dLk = dLm2 + (2.0 * rk - 1.0) * Lm1;
+    q = Lk - Lm2;
+    dq = dLk - dLm2;
+    LN = L;
+}
+
+// LGL nodes and weights: Newton on q = L_{N+1} - L_{N-1} from Parter's
+// asymptotic guess (utils/interpol_mod.f90:513-582).
+void lgl(int N, double* x, double* w) {
+    const double pi = 4.0 * std::atan(1.0);
+    double q, dq, LN = 0.0;
+    if (N == 1) {
+        x[0] = -1.0; x[1] = 1.0; w[0] = w[1] = 1.0;
+        return;
+    }
+    x[0] = -1.0;
+    x[N] = 1.0;
+    w[0] = w[N] = 2.0 / (double(N) * (double(N) + 1));
+    for (int j = 1; j <= (N + 1) / 2 - 1; ++j) {
+        const double rj = double(j), rN = double(N);
+        x[j] = -std::cos(((rj + 0.25) * pi) / rN - (3.0 / (8.0 * rN * pi)) * (1.0 / (rj + 0.25)));
+        for (int it = 0; it <= 4; ++it) {
+            q_and_L(N, x[j], q, dq, LN);
+            const double d = -q / dq;
+            x[j] += d;
+            if (std::fabs(d) <= 4.0e-16 * std::fabs(x[j])) break;
+        }
+        q_and_L(N, x[j], q, dq, LN);
+        x[N - j] = -x[j];
+        w[j] = w[N - j] = 2.0 / (double(N) * (double(N) + 1.0) * (LN * LN));
+    }
+    if (N % 2 == 0) {
+        q_and_L(N, 0.0, q, dq, LN);
+        x[N / 2] = 0.0;
+        w[N / 2] = 2.0 / (double(N) * (double(N) + 1.0) * (LN * LN));
+    }
+}
+
+// Barycentric differentiation matrix (utils/interpol_mod.f90:38-80).
+void diff_matrix(int N, const double* x, double* D) {
+    const int M = N + 1;
+    double wb[kMaxM];
+    for (int i = 0; i < M; ++i) wb[i] = 1.0;
+    for (int i = 1; i < M; ++i)
+        for (int j = 0; j < i; ++j) {
+            wb[j] *= (x[j] - x[i]);
+            wb[i] *= (x[i] - x[j]);
+        }
+    for (int i = 0; i < M; ++i) wb[i] = 1.0 / wb[i];
+    for (int i = 0; i < M * M; ++i) D[i] = 0.0;
+    for (int l = 0; l < M; ++l)
+        for (int g = 0; g < M; ++g)
+            if (l != g) {
+                D[g * M + l] = wb[l] / (wb[g] * (x[g] - x[l]));
+                D[g * M + g] -= D[g * M + l];
+            }
+}
+
+// Normalised Legendre polynomial (utils/interpol_mod.f90:274-311).
+double legendre_hat(int n, double x) {
+    double L;
+    if (n == 0) {
+        L = 1.0;
+    } else if (n == 1) {
+        L = x;
+    } else {
+        double Lm2 = 1.0, Lm1 = x;
+        L = 0.0;
+        for (int i = 2; i <= n; ++i) {
+            L = (double(2 * i - 1) * x * Lm1 - double(i - 1) * Lm2) / double(i);
+            Lm2 = Lm1;
+            Lm1 = L;
+        }
+    }
+    return L * std::sqrt(double(n) + 0.5);
+}
+
+// Inverse by Doolittle LU without pivoting (utils/linalg_mod.f90:123-200).
+void invert(int n, const double* A, double* C) {
+    double a[kMaxM * kMaxM], L[kMaxM * kMaxM] = {0}, U[kMaxM * kMaxM] = {0};
+    std::memcpy(a, A, sizeof(double) * n * n);
+    for (int k = 0; k < n - 1; ++k)
+        for (int i = k + 1; i < n; ++i) {
+            const double coeff = a[i * n + k] / a[k * n + k];
+            L[i * n + k] = coeff;
+            for (int j = k + 1; j < n; ++j) a[i * n + j] -= coeff * a[k * n + j];
+        }
+    for (int i = 0; i < n; ++i) L[i * n + i] = 1.0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i <= j; ++i) U[i * n + j] = a[i * n + j];
+    for (int k = 0; k < n; ++k) {
+        double d[kMaxM], x[kMaxM];
+        for (int i = 0; i < n; ++i) {
+            d[i] = (i == k) ? 1.0 : 0.0;
+            for (int j = 0; j < i; ++j) d[i] -= L[i * n + j] * d[j];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            x[i] = d[i];
+            for (int j = n - 1; j > i; --j) x[i] -= U[i * n + j] * x[j];
+            x[i] /= U[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) C[i * n + k] = x[i];
+    }
+}
+
+struct HostOps {
+    int N, M;
+    double x[kMaxM], w[kMaxM], D[kMaxM * kMaxM], Vinv[kMaxM * kMaxM];
+};
+
+void build_host_ops(int N, HostOps& op) {
+    op.N = N;
+    op.M = N + 1;
+    const int M = op.M;
+    lgl(N, op.x, op.w);
+    diff_matrix(N, op.x, op.D);
+    double V[kMaxM * kMaxM];
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) V[i * M + j] = legendre_hat(j, op.x[i]);
+    invert(M, V, op.Vinv);
+}
+
+template <int M, typename Real>
+nemo::OpTable<M, Real> make_table(const HostOps& h) {
+    nemo::OpTable<M, Real> t;
+    for (int i = 0; i < M; ++i) {
+        t.w[i] = Real(h.w[i]);
+        t.inv_w[i] = Real(1.0 / h.w[i]);
+        // weak-form surface lifting on LGL nodes: 2/N_NODES/w_i * l_i(+-1)
+        // (kernel/dgsem/weakform/2d/kernel_operators_mod.f90:656-687)
+        t.lagrM[i] = Real(i == 0 ? 2.0 / double(M) / h.w[i] : 0.0);
+        t.lagrP[i] = Real(i == M - 1 ? 2.0 / double(M) / h.w[i] : 0.0);
+        for (int j = 0; j < M; ++j) {
+            t.S[i][j] = Real(i == j ? 0.0 : 2.0 * h.D[i * M + j]);
+            // weak-form volume matrix: -(-2 w_j/w_i D_ji)/N_NODES
+            // (kernel_operators_mod.f90:94-126)
+            const double temp = -2.0 * h.w[j] / h.w[i] * h.D[j * M + i];
+            t.dgDiff[i][j] = Real(-temp / double(M));
+            t.Vinv[i][j] = Real(h.Vinv[i * M + j]);
+        }
+    }
+    return t;
+}
+
+nemo::RKCoeffs make_rk() {
+    nemo::RKCoeffs rk;
+    // low-storage/4-5/timedisc_const_mod.f90:5-21 (a_rk, c_rk)
+    const double a[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                         -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+    const double c[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                         1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                         2277821191437.0 / 14882151754819.0};
+    // ssp/4-5/timedisc_const_mod.f90:5-32
+    const double s[23] = {1.00000000000000, 1.00000000000000, 0.00000000000000, 0.39175222700392,
+                          0.39175222700392, 0.44437049406734, 0.55562950593266, 0.36841059262959,
+                          0.58607968896780, 0.62010185138540, 0.37989814861460, 0.25189177424738,
+                          0.47454236302687, 0.17807995410773, 0.82192004589227, 0.54497475021237,
+                          0.93501063100924, 0.00683325884039, 0.34833675773694, 0.22600748319395,
+                          0.51723167208978, 0.12759831133288, 0.08460416338212};
+    for (int i = 0; i < 5; ++i) {
+        rk.lsrk_a[i] = a[i];
+        rk.lsrk_c[i] = c[i];
+    }
+    for (int i = 0; i < 23; ++i) rk.ssp[i] = s[i];
+    return rk;
+}
+
+// ---------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------
+
+struct Solver {
+    nemo_gpu_config cfg{};
+    HostOps ops{};
+    nemo::RKCoeffs rk{};
+    int M = 0, M3 = 0, V = 5;
+    long long nelem = 0;
+    bool have_mesh = false, have_state = false;
+
+    double* u[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* r0 = nullptr;
+    double* r1 = nullptr;
+    double* rhs = nullptr;     // RHS / scratch of state size
+    double* alpha = nullptr;   // [nelem]
+    int32_t* nbr = nullptr;
+    double* delta = nullptr;
+    double* dt_dev = nullptr;  // [3]: dt, lmax, vmax
+    unsigned long long* red = nullptr;  // [3]
+    int* status = nullptr;
+    double* hist = nullptr;
+    long long hist_cap = 0;
+    hipStream_t stream = nullptr;
+
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+
+    size_t state_doubles() const { return size_t(V) * size_t(nelem) * size_t(M3); }
+};
+
+void free_mesh_buffers(Solver& s) {
+    (void)hipFree(s.u[0]);
+    (void)hipFree(s.u[1]);
+    (void)hipFree(s.r0);
+    (void)hipFree(s.r1);
+    (void)hipFree(s.rhs);
+    (void)hipFree(s.alpha);
+    (void)hipFree(s.nbr);
+    (void)hipFree(s.delta);
+    s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = nullptr;
+    s.nbr = nullptr;
+}
+
+// Elements per workgroup: one wavefront of lines (M*M lines per element).
+template <int M>
+struct Tile;
+template <>
+struct Tile<3> { static constexpr int BE = 7; };
+template <>
+struct Tile<4> { static constexpr int BE = 4; };
+template <>
+struct Tile<5> { static constexpr int BE = 2; };
+
+template <int M>
+nemo::StageArgs<M, double> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
+                                     double* alpha_out) {
+    nemo::StageArgs<M, double> a{};
+    a.u_in = s.u[s.cur];
+    a.u_out = s.u[s.cur ^ 1];
+    a.r0 = s.r0;
+    a.r1 = s.r1;
+    a.rhs_out = rhs_out;
+    a.alpha_out = alpha_out;
+    a.nbr = s.nbr;
+    a.delta = s.delta;
+    a.dt_dev = dt_dev;
+    a.status = s.status;
+    a.nelem = s.nelem;
+    a.dt = dt;
+    a.stage = stage;
+    a.timedisc = s.cfg.timedisc;
+    a.indicator = s.cfg.indicator;
+    a.repair = s.cfg.repair;
+    a.alpha_max = s.cfg.alpha_max;
+    a.alpha_min = s.cfg.alpha_min;
+    a.ind_threshold = 0.5 * std::pow(10.0, -1.8 * std::pow(double(M), 0.25));
+    a.ind_s = std::log((1.0 - 0.0001) / 0.0001);
+    const double k = s.cfg.gamma;
+    a.gas.kappa = k;
+    a.gas.kappa_m1 = k - 1.0;
+    a.gas.hllc_beta = std::sqrt(0.5 * (k - 1) / k);
+    a.gas.half_inv_km1 = 0.5 / (k - 1.0);
+    a.op = make_table<M, double>(s.ops);
+    a.rk = s.rk;
+    return a;
+}
+
+template <int M, int VOL, int SURF>
+void launch_stage_t(Solver& s, const nemo::StageArgs<M, double>& a) {
+    constexpr int BE = Tile<M>::BE;
+    using Sh = nemo::StageShape<M, BE>;
+    const long long nblocks = (s.nelem + BE - 1) / BE;
+    hipLaunchKernelGGL((nemo::stage_kernel<M, BE, VOL, SURF>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0, s.stream, a);
+}
+
+template <int M>
+int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out) {
+    const auto a = make_args<M>(s, stage, dt, dt_dev, rhs_out, alpha_out);
+    const int vol = s.cfg.volume, surf = s.cfg.surface;
+#define NEMO_CASE(V_, S_)                              \
+    if (vol == V_ && surf == S_) {                     \
+        launch_stage_t<M, V_, S_>(s, a);               \
+        return NEMO_OK;                                \
+    }
+    NEMO_CASE(nemo::VOL_WEAK, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::VOL_WEAK, nemo::SURF_HLLC)
+    NEMO_CASE(nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::VOL_CENTRAL, nemo::SURF_HLLC)
+    NEMO_CASE(nemo::VOL_EC, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::VOL_EC, nemo::SURF_HLLC)
+#undef NEMO_CASE
+    return fail(NEMO_ERR_ARG, "unsupported volume/surface combination");
+}
+
+int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s.timing) {
+        if (s.ev_used == s.ev.size()) {
+            hipEvent_t a, b;
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            s.ev.emplace_back(a, b);
+        }
+        e0 = s.ev[s.ev_used].first;
+        e1 = s.ev[s.ev_used].second;
+        ++s.ev_used;
+        HIP_TRY(hipEventRecord(e0, s.stream));
+    }
+    int rc;
+    switch (s.M) {
+    case 3: rc = launch_stage_m<3>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
+    case 4: rc = launch_stage_m<4>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
+    case 5: rc = launch_stage_m<5>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
+    default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    }
+    if (rc != NEMO_OK) return rc;
+    HIP_TRY(hipGetLastError());
+    if (s.timing) HIP_TRY(hipEventRecord(e1, s.stream));
+    return NEMO_OK;
+}
+
+template <int M>
+void launch_cfl_m(Solver& s) {
+    constexpr int BE = (M == 3) ? 8 : (M == 4 ? 4 : 2);
+    nemo::Gas<double> g;
+    const double k = s.cfg.gamma;
+    g.kappa = k;
+    g.kappa_m1 = k - 1.0;
+    g.hllc_beta = std::sqrt(0.5 * (k - 1) / k);
+    g.half_inv_km1 = 0.5 / (k - 1.0);
+    const long long nblocks = (s.nelem + BE - 1) / BE;
+    hipLaunchKernelGGL((nemo::cfl_kernel<M, BE>), dim3(unsigned(nblocks)), dim3(BE * M * M * M), 0, s.stream,
+                       s.u[s.cur], s.delta, s.nelem, g, s.red);
+}
+
+int launch_cfl(Solver& s, long long hist_slot) {
+    switch (s.M) {
+    case 3: launch_cfl_m<3>(s); break;
+    case 4: launch_cfl_m<4>(s); break;
+    case 5: launch_cfl_m<5>(s); break;
+    default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(nemo::cfl_finalize, dim3(1), dim3(64), 0, s.stream, s.red, s.cfg.cfl, s.M, s.dt_dev,
+                       hist_slot >= 0 ? s.hist : nullptr, hist_slot);
+    HIP_TRY(hipGetLastError());
+    return NEMO_OK;
+}
+
+int check_ready(Solver* s) {
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    if (!s->have_mesh) return fail(NEMO_ERR_NOT_READY, "mesh not uploaded");
+    if (!s->have_state) return fail(NEMO_ERR_NOT_READY, "state not uploaded");
+    return NEMO_OK;
+}
+
+// host layout u[e][v][n]  <->  device layout u[v][e][n]
+void to_device_layout(const Solver& s, const double* host, std::vector<double>& dev) {
+    dev.resize(s.state_doubles());
+    const size_t M3 = s.M3, E = s.nelem;
+    for (size_t e = 0; e < E; ++e)
+        for (int v = 0; v < s.V; ++v)
+            std::memcpy(&dev[(v * E + e) * M3], &host[(e * s.V + v) * M3], M3 * sizeof(double));
+}
+
+void to_host_layout(const Solver& s, const std::vector<double>& dev, double* host) {
+    const size_t M3 = s.M3, E = s.nelem;
+    for (size_t e = 0; e < E; ++e)
+        for (int v = 0; v < s.V; ++v)
+            std::memcpy(&host[(e * s.V + v) * M3], &dev[(v * E + e) * M3], M3 * sizeof(double));
+}
+
+int status_check(Solver* s) {
+    int code = 0;
+    HIP_TRY(hipMemcpyAsync(&code, s->status, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (code) {
+        HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        return fail(NEMO_ERR_STATE, "positivity repair failed (kernel_repair returned false)");
+    }
+    return NEMO_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int nemo_gpu_error_message(char* buf, int32_t len) {
+    if (!buf || len <= 0) return NEMO_ERR_ARG;
+    std::snprintf(buf, size_t(len), "%s", g_err.c_str());
+    return NEMO_OK;
+}
+
+int nemo_gpu_operators(int32_t n_poly, double* x, double* w, double* D, double* Vinv) {
+    if (n_poly < 1 || n_poly + 1 > kMaxM) return fail(NEMO_ERR_ARG, "n_poly out of range");
+    HostOps op;
+    build_host_ops(n_poly, op);
+    const int M = op.M;
+    if (x) std::memcpy(x, op.x, sizeof(double) * M);
+    if (w) std::memcpy(w, op.w, sizeof(double) * M);
+    if (D) std::memcpy(D, op.D, sizeof(double) * M * M);
+    if (Vinv) std::memcpy(Vinv, op.Vinv, sizeof(double) * M * M);
+    return NEMO_OK;
+}
+
+int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
+    if (!cfg || !handle) return fail(NEMO_ERR_ARG, "null argument");
+    if (cfg->n_poly < 2 || cfg->n_poly > 4) return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    if (cfg->equations != NEMO_EQ_EULER) return fail(NEMO_ERR_ARG, "equations: only NEMO_EQ_EULER is built");
+    if (cfg->volume < 0 || cfg->volume > 2) return fail(NEMO_ERR_ARG, "invalid volume flux");
+    if (cfg->surface < 0 || cfg->surface > 1) return fail(NEMO_ERR_ARG, "invalid surface flux");
+    if (cfg->indicator < 0 || cfg->indicator > 1) return fail(NEMO_ERR_ARG, "invalid indicator");
+    if (cfg->timedisc < 0 || cfg->timedisc > 1) return fail(NEMO_ERR_ARG, "invalid timedisc");
+    if (cfg->precision != 0) return fail(NEMO_ERR_ARG, "precision: only the FP64 operator is built");
+    if (!(cfg->gamma > 1.0)) return fail(NEMO_ERR_ARG, "gamma must exceed 1");
+    Solver* s = new Solver();
+    s->cfg = *cfg;
+    s->M = cfg->n_poly + 1;
+    s->M3 = s->M * s->M * s->M;
+    s->V = 5;
+    build_host_ops(cfg->n_poly, s->ops);
+    s->rk = make_rk();
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(NEMO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (hipMalloc(&s->dt_dev, 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s->red, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return fail(NEMO_ERR_HIP, "hipMalloc failed");
+    }
+    const unsigned long long init[4] = {0x7FEFFFFFFFFFFFFFull, 0ull, 0ull, 0ull};
+    (void)hipMemcpy(s->red, init, sizeof(init), hipMemcpyHostToDevice);
+    (void)hipMemset(s->status, 0, sizeof(int));
+    *handle = s;
+    return NEMO_OK;
+}
+
+int nemo_gpu_destroy(void* handle) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return NEMO_OK;
+    (void)hipStreamSynchronize(s->stream);
+    free_mesh_buffers(*s);
+    (void)hipFree(s->dt_dev);
+    (void)hipFree(s->red);
+    (void)hipFree(s->status);
+    (void)hipFree(s->hist);
+    for (auto& p : s->ev) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return NEMO_OK;
+}
+
+int nemo_gpu_upload_mesh(void* handle, int64_t nelem, const int32_t* nbr, const double* delta) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !nbr || !delta || nelem <= 0) return fail(NEMO_ERR_ARG, "invalid mesh arguments");
+    for (int64_t i = 0; i < 6 * nelem; ++i)
+        if (nbr[i] < 0 || nbr[i] >= nelem) return fail(NEMO_ERR_ARG, "neighbour index out of range");
+    for (int64_t i = 0; i < 3 * nelem; ++i)
+        if (!(delta[i] > 0.0)) return fail(NEMO_ERR_ARG, "element size must be positive");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (nelem != s->nelem) {
+        free_mesh_buffers(*s);
+        s->nelem = nelem;
+        const size_t bytes = s->state_doubles() * sizeof(double);
+        HIP_TRY(hipMalloc(&s->u[0], bytes));
+        HIP_TRY(hipMalloc(&s->u[1], bytes));
+        HIP_TRY(hipMalloc(&s->r0, bytes));
+        HIP_TRY(hipMalloc(&s->r1, bytes));
+        HIP_TRY(hipMalloc(&s->rhs, bytes));
+        HIP_TRY(hipMalloc(&s->alpha, size_t(nelem) * sizeof(double)));
+        HIP_TRY(hipMalloc(&s->nbr, size_t(nelem) * 6 * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&s->delta, size_t(nelem) * 3 * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(s->r0, 0, bytes, s->stream));
+        HIP_TRY(hipMemsetAsync(s->r1, 0, bytes, s->stream));
+        s->have_state = false;
+    }
+    HIP_TRY(hipMemcpyAsync(s->nbr, nbr, size_t(nelem) * 6 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->delta, delta, size_t(nelem) * 3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->have_mesh = true;
+    return NEMO_OK;
+}
+
+int nemo_gpu_upload_state(void* handle, const double* u) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !u) return fail(NEMO_ERR_ARG, "null argument");
+    if (!s->have_mesh) return fail(NEMO_ERR_NOT_READY, "mesh not uploaded");
+    std::vector<double> dev;
+    to_device_layout(*s, u, dev);
+    const size_t bytes = s->state_doubles() * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(s->u[s->cur], dev.data(), bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->r0, 0, bytes, s->stream));
+    HIP_TRY(hipMemsetAsync(s->r1, 0, bytes, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->have_state = true;
+    return NEMO_OK;
+}
+
+int nemo_gpu_download_state(void* handle, double* u) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!u) return fail(NEMO_ERR_ARG, "null argument");
+    std::vector<double> dev(s->state_doubles());
+    HIP_TRY(hipMemcpyAsync(dev.data(), s->u[s->cur], dev.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    to_host_layout(*s, dev, u);
+    return NEMO_OK;
+}
+
+int nemo_gpu_rhs(void* handle, double* rhs) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!rhs) return fail(NEMO_ERR_ARG, "null argument");
+    if (int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, nullptr)) return rc;
+    std::vector<double> dev(s->state_doubles());
+    HIP_TRY(hipMemcpyAsync(dev.data(), s->rhs, dev.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    to_host_layout(*s, dev, rhs);
+    return NEMO_OK;
+}
+
+int nemo_gpu_indicator(void* handle, double* alpha) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!alpha) return fail(NEMO_ERR_ARG, "null argument");
+    if (s->cfg.indicator == NEMO_IND_NONE) {
+        for (long long e = 0; e < s->nelem; ++e) alpha[e] = 0.0;
+        return NEMO_OK;
+    }
+    if (int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, s->alpha)) return rc;
+    HIP_TRY(hipMemcpyAsync(alpha, s->alpha, size_t(s->nelem) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_calc_timestep(void* handle, double* dt, double* lmax, double* vmax) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (int rc = launch_cfl(*s, -1)) return rc;
+    double h[3];
+    HIP_TRY(hipMemcpyAsync(h, s->dt_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (dt) *dt = h[0];
+    if (lmax) *lmax = h[1];
+    if (vmax) *vmax = h[2];
+    return NEMO_OK;
+}
+
+int nemo_gpu_stage(void* handle, int32_t stage, double dt) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (stage < 1 || stage > 5) return fail(NEMO_ERR_ARG, "stage must be 1..5");
+    if (int rc = launch_stage(*s, stage, dt, nullptr, nullptr, nullptr)) return rc;
+    s->cur ^= 1;
+    return NEMO_OK;
+}
+
+int nemo_gpu_step(void* handle, double dt) {
+    for (int st = 1; st <= 5; ++st)
+        if (int rc = nemo_gpu_stage(handle, st, dt)) return rc;
+    return NEMO_OK;
+}
+
+int nemo_gpu_run(void* handle, int32_t nsteps, double* dt_hist) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (nsteps < 0) return fail(NEMO_ERR_ARG, "nsteps must be >= 0");
+    if (dt_hist && s->hist_cap < nsteps) {
+        (void)hipFree(s->hist);
+        s->hist = nullptr;
+        HIP_TRY(hipMalloc(&s->hist, size_t(nsteps) * sizeof(double)));
+        s->hist_cap = nsteps;
+    }
+    for (int it = 0; it < nsteps; ++it) {
+        if (int rc = launch_cfl(*s, dt_hist ? it : -1)) return rc;
+        for (int st = 1; st <= 5; ++st) {
+            if (int rc = launch_stage(*s, st, 0.0, s->dt_dev, nullptr, nullptr)) return rc;
+            s->cur ^= 1;
+        }
+    }
+    if (dt_hist && nsteps > 0)
+        HIP_TRY(hipMemcpyAsync(dt_hist, s->hist, size_t(nsteps) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return status_check(s);
+}
+
+int nemo_gpu_synchronize(void* handle) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return status_check(s);
+}
+
+int nemo_gpu_get_status(void* handle, int32_t* code) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !code) return fail(NEMO_ERR_ARG, "null argument");
+    int c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, s->status, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *code = c;
+    return NEMO_OK;
+}
+
+int nemo_gpu_set_timing(void* handle, int32_t enable) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->timing = enable != 0;
+    s->ev_used = 0;
+    return NEMO_OK;
+}
+
+int nemo_gpu_read_timing(void* handle, int64_t* launches, double* total_ms) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !launches || !total_ms) return fail(NEMO_ERR_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double tot = 0.0;
+    for (size_t i = 0; i < s->ev_used; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev[i].first, s->ev[i].second));
+        tot += double(ms);
+    }
+    *launches = int64_t(s->ev_used);
+    *total_ms = tot;
+    s->ev_used = 0;
+    return NEMO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""Host-side mirror of nemo's kernel / timedisc / driver interfaces over the
+C-ABI of libnemo_gpu.so.
+
+The reference drives one time step as (driver_mod.f90:62-150):
+
+    hook_calc_timestep  -> CFL reduction            (kernel_timestep_mod.f90:19-63)
+    hook_evolve_500     -> for stage in 1..nstages:  (timedisc_evolve_mod.f90:20-84)
+                             rktimestep = dt*dt_coeffs(stage)
+                             hook_pre_rk_timestep   (halo exchange, face fluxes)
+                             hook_mid_rk_timestep   (rkstep_cb per quad)
+                             status check
+
+The class below keeps those names and their meaning; each hook is one C-ABI
+call.  Errors follow the reference's convention: a failed call sets
+``runtime.stat`` like ``raise_error`` (utils/status_mod.f90:7-21) and raises
+``NemoGpuError``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, NemoGpuError, check, dptr, iptr, lib
+from .mesh import Mesh
+
+# timedisc_const_mod::dt_coeffs (low-storage/4-5:29 = b_rk, ssp/4-5:36)
+LSRK54_DT_COEFFS = (0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                    2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0)
+SSPRK54_DT_COEFFS = (1.0, 0.39175222700392, 0.58607968896780, 0.47454236302687, 0.93501063100924)
+NSTAGES = 5
+
+
+@dataclass
+class Status:
+    """status_t of runtime_mod.f90:55-61."""
+    ok: bool = True
+    errcode: int = 0
+    message: str = ""
+
+
+@dataclass
+class Runtime:
+    """runtime_t of runtime_mod.f90:63-84 (the fields the hot path uses)."""
+    simtime: float = 0.0
+    timestep: float = 0.0
+    maxtimestep: float = 0.0
+    rktimestep: float = 0.0
+    rksimtime: float = 0.0
+    rkstage: int = 0
+    timesteps: int = 0
+    stat: Status = field(default_factory=Status)
+
+
+def make_config(N: int = 3, volume: int = _lib.VOL_WEAK, surface: int = _lib.SURF_RUSANOV,
+                indicator: int = _lib.IND_NONE, timedisc: int = _lib.TD_SSPRK54, repair: int = 0,
+                gamma: float = 1.4, alpha_max: float = 0.5, alpha_min: float = 0.001, cfl: float = 0.5,
+                precision: int = 0) -> Config:
+    return Config(N, _lib.EQ_EULER, volume, surface, indicator, timedisc, repair, precision, gamma,
+                  alpha_max, alpha_min, cfl, 1.0)
+
+
+class Solver:
+    """One device-resident solver (an opaque nemo_gpu handle)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.M = cfg.n_poly + 1
+        self.M3 = self.M ** 3
+        self.nvars = 5
+        self.runtime = Runtime()
+        self._h = ctypes.c_void_p()
+        self._call("nemo_gpu_create", ctypes.byref(cfg), ctypes.byref(self._h))
+        self.nelem = 0
+
+    # -- plumbing ----------------------------------------------------------
+    def _call(self, name: str, *args) -> None:
+        rc = getattr(lib(), name)(*args)
+        if rc != _lib.NEMO_OK:
+            msg = _lib.last_error()
+            self.runtime.stat.ok = False
+            self.runtime.stat.errcode = -1
+            self.runtime.stat.message += msg + "\n"
+            raise NemoGpuError(rc, name, msg)
+
+    def close(self) -> None:
+        if self._h:
+            lib().nemo_gpu_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- mesh and state (mesh_alloc / payload upload) -----------------------
+    def upload_mesh(self, mesh: Mesh) -> None:
+        nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32)
+        delta = np.ascontiguousarray(mesh.delta, dtype=np.float64)
+        self._call("nemo_gpu_upload_mesh", self._h, ctypes.c_int64(mesh.nelem), iptr(nbr), dptr(delta))
+        self.nelem = mesh.nelem
+
+    def set_state(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (self.nelem, self.nvars, self.M3), u.shape
+        self._call("nemo_gpu_upload_state", self._h, dptr(u))
+
+    def get_state(self) -> np.ndarray:
+        u = np.empty((self.nelem, self.nvars, self.M3))
+        self._call("nemo_gpu_download_state", self._h, dptr(u))
+        return u
+
+    # -- kernel interface ----------------------------------------------------
+    def kernel_calc_flux(self) -> np.ndarray:
+        """du/dt of the current state (kernel_calc_flux after the face pass)."""
+        r = np.empty((self.nelem, self.nvars, self.M3))
+        self._call("nemo_gpu_rhs", self._h, dptr(r))
+        return r
+
+    def indicator(self) -> np.ndarray:
+        a = np.empty(self.nelem)
+        self._call("nemo_gpu_indicator", self._h, dptr(a))
+        return a
+
+    def hook_calc_timestep(self):
+        """CFL time step; sets runtime.maxtimestep (kernel_timestep_mod.f90:19-63)."""
+        dt, lm, vm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._call("nemo_gpu_calc_timestep", self._h, ctypes.byref(dt), ctypes.byref(lm), ctypes.byref(vm))
+        self.runtime.maxtimestep = dt.value
+        return dt.value, lm.value, vm.value
+
+    # -- timedisc interface ----------------------------------------------------
+    def dt_coeffs(self):
+        return LSRK54_DT_COEFFS if self.cfg.timedisc == _lib.TD_LSRK54 else SSPRK54_DT_COEFFS
+
+    def hook_evolve(self) -> None:
+        """All stages of one step with runtime.timestep (timedisc_evolve_mod.f90:20-84)."""
+        rt = self.runtime
+        for stage in range(1, NSTAGES + 1):
+            rt.rkstage = stage
+            rt.rktimestep = rt.timestep * self.dt_coeffs()[stage - 1]
+            rt.rksimtime = rt.simtime + rt.rktimestep
+            self._call("nemo_gpu_stage", self._h, ctypes.c_int32(stage), ctypes.c_double(rt.timestep))
+        self._call("nemo_gpu_synchronize", self._h)
+
+    def step(self, dt: float) -> None:
+        self._call("nemo_gpu_step", self._h, ctypes.c_double(dt))
+
+    def run(self, nsteps: int, record_dt: bool = False) -> Optional[np.ndarray]:
+        """nsteps CFL-limited steps entirely on the device."""
+        hist = np.zeros(max(nsteps, 1)) if record_dt else None
+        self._call("nemo_gpu_run", self._h, ctypes.c_int32(nsteps), dptr(hist) if record_dt else None)
+        self.runtime.timesteps += nsteps
+        return hist[:nsteps] if record_dt else None
+
+    def synchronize(self) -> None:
+        self._call("nemo_gpu_synchronize", self._h)
+
+    # -- timing ------------------------------------------------------------------
+    def set_timing(self, enable: bool) -> None:
+        self._call("nemo_gpu_set_timing", self._h, ctypes.c_int32(1 if enable else 0))
+
+    def read_timing(self):
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        self._call("nemo_gpu_read_timing", self._h, ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
+    # -- driver loop -------------------------------------------------------------
+    def mainloop(self, stop_time: float, max_steps: int = 1 << 30) -> Runtime:
+        """Time loop of driver_mod.f90:62-150 without checkpoint/analysis:
+        dt = min(maxtimestep, |stop_time - simtime|)."""
+        rt = self.runtime
+        tol = 4.0e-16
+        while rt.stat.ok and rt.timesteps < max_steps:
+            self.hook_calc_timestep()
+            if abs(rt.simtime - stop_time) < 100 * tol:
+                break
+            rt.timestep = min(rt.maxtimestep, abs(stop_time - rt.simtime))
+            self.hook_evolve()
+            rt.simtime += rt.timestep
+            rt.timesteps += 1
+        return rt

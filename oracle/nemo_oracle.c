@@ -1,0 +1,936 @@
+/*
+ * nemo_oracle.c -- FP64 CPU restatement of nemo's spatial operator and RK
+ * stage update.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker for the HIP product, never
+ * the thing measured or shipped.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference's source/ directory).  The 3-D LGL flux-differencing
+ * operator, the LGL-subcell FV operator and the Persson-Peraire indicator
+ * have no implementation in the reference snapshot; they are restated from
+ * the literature (Gassner/Winters/Kopriva 2016, Hennemann et al. JCP 2021,
+ * Persson/Peraire 2006) and pinned by the invariants in tests/.
+ *
+ * Built with -ffp-contract=off so that every expression is evaluated in the
+ * order written, as the reference's Fortran is without -ffast-math.
+ */
+#include "nemo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXM 8
+#define MAXV 9
+
+enum { DENS = 0, MOMX = 1, MOMY = 2, MOMZ = 3, ENER = 4 };
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int orc_nvars(const orc_cfg *c) { return c->eq == ORC_EQ_MHD ? 9 : 5; }
+
+/* ======================================================================== */
+/* Operators                                                                */
+/* ======================================================================== */
+
+/* qAndLEvaluation: utils/interpol_mod.f90:468-509 */
+static void q_and_l(int N, double x, double *q, double *dq, double *LN) {
+    double L_N_2 = 1.0, L_N_1 = x, dL_N_2 = 0.0, dL_N_1 = 1.0;
+    double L_N = 0.0, dL_N = 0.0;
+    for (int k = 2; k <= N; ++k) {
+        double rk = (double)k;
+        L_N = (2.0 * rk - 1.0) / rk * x * L_N_1 - (rk - 1.0) / rk * L_N_2;
+        dL_N = dL_N_2 + (2.0 * rk - 1.0) * L_N_1;
+        L_N_2 = L_N_1;
+        L_N_1 = L_N;
+        dL_N_2 = dL_N_1;
+        dL_N_1 = dL_N;
+    }
+    int k = N + 1;
+    double rk = (double)k;
+    double L_k = (2.0 * rk - 1.0) / rk * x * L_N - (rk - 1.0) / rk * L_N_2;
+    double dL_k = dL_N_2 + (2.0 * rk - 1.0) * L_N_1;
+    *q = L_k - L_N_2;
+    *dq = dL_k - dL_N_2;
+    *LN = L_N;
+}
+
+/* LegendreGaussLobattoNodesAndWeights: utils/interpol_mod.f90:513-582 */
+void orc_lgl_nodes_weights(int N, double *x, double *w) {
+    const double pi = 4.0 * atan(1.0);
+    const double tol = 4.0e-16;
+    double q, dq, L_N = 0.0;
+    if (N == 1) {
+        x[0] = -1.0; w[0] = 1.0; x[1] = 1.0; w[1] = w[0];
+        return;
+    }
+    x[0] = -1.0;
+    w[0] = 2.0 / ((double)N * ((double)N + 1));
+    x[N] = 1.0;
+    w[N] = w[0];
+    for (int j = 1; j <= (N + 1) / 2 - 1; ++j) {
+        double rj = (double)j, rN = (double)N;
+        x[j] = -cos(((rj + 0.25) * pi) / rN - (3.0 / (8.0 * rN * pi)) * (1.0 / (rj + 0.25)));
+        for (int k = 0; k <= 4; ++k) {
+            q_and_l(N, x[j], &q, &dq, &L_N);
+            double delta = -q / dq;
+            x[j] = x[j] + delta;
+            if (fabs(delta) <= tol * fabs(x[j])) break;
+        }
+        q_and_l(N, x[j], &q, &dq, &L_N);
+        x[N - j] = -x[j];
+        w[j] = 2.0 / ((double)N * ((double)N + 1.0) * (L_N * L_N));
+        w[N - j] = w[j];
+    }
+    if (N % 2 == 0) {
+        q_and_l(N, 0.0, &q, &dq, &L_N);
+        x[N / 2] = 0.0;
+        w[N / 2] = 2.0 / ((double)N * ((double)N + 1.0) * (L_N * L_N));
+    }
+}
+
+/* BarycentricWeights + DiffMatrix: utils/interpol_mod.f90:38-80.
+ * D[i*M + j] = D(iGP=i, iLagrange=j) = l_j'(x_i). */
+void orc_diff_matrix(int N, const double *x, double *D) {
+    int M = N + 1;
+    double wb[MAXM];
+    for (int i = 0; i <= N; ++i) wb[i] = 1.0;
+    for (int i = 1; i <= N; ++i)
+        for (int j = 0; j <= i - 1; ++j) {
+            wb[j] = wb[j] * (x[j] - x[i]);
+            wb[i] = wb[i] * (x[i] - x[j]);
+        }
+    for (int i = 0; i <= N; ++i) wb[i] = 1.0 / wb[i];
+    for (int i = 0; i < M * M; ++i) D[i] = 0.0;
+    for (int l = 0; l <= N; ++l)
+        for (int g = 0; g <= N; ++g)
+            if (l != g) {
+                D[g * M + l] = wb[l] / (wb[g] * (x[g] - x[l]));
+                D[g * M + g] = D[g * M + g] - D[g * M + l];
+            }
+}
+
+/* LegendrePolynomialAndDerivative (normalised): utils/interpol_mod.f90:274-311 */
+void orc_legendre(int n, double x, double *L, double *dL) {
+    if (n == 0) {
+        *L = 1.0; *dL = 0.0;
+    } else if (n == 1) {
+        *L = x; *dL = 1.0;
+    } else {
+        double Lm2 = 1.0, Lm1 = x, dLm2 = 0.0, dLm1 = 1.0, l = 0.0, dl = 0.0;
+        for (int i = 2; i <= n; ++i) {
+            l = ((double)(2 * i - 1) * x * Lm1 - (double)(i - 1) * Lm2) / (double)i;
+            dl = dLm2 + (double)(2 * i - 1) * Lm1;
+            Lm2 = Lm1; Lm1 = l; dLm2 = dLm1; dLm1 = dl;
+        }
+        *L = l; *dL = dl;
+    }
+    *L = *L * sqrt((double)n + 0.5);
+    *dL = *dL * sqrt((double)n + 0.5);
+}
+
+/* invert (Doolittle LU, no pivoting): utils/linalg_mod.f90:123-200.
+ * Row-major a[i*n+j]. */
+void orc_invert(int n, const double *o, double *c) {
+    double a[MAXM * MAXM], L[MAXM * MAXM], U[MAXM * MAXM], b[MAXM], d[MAXM], xx[MAXM];
+    memcpy(a, o, sizeof(double) * n * n);
+    for (int i = 0; i < n * n; ++i) { L[i] = 0.0; U[i] = 0.0; }
+    for (int i = 0; i < n; ++i) b[i] = 0.0;
+    for (int k = 0; k < n - 1; ++k)
+        for (int i = k + 1; i < n; ++i) {
+            double coeff = a[i * n + k] / a[k * n + k];
+            L[i * n + k] = coeff;
+            for (int j = k + 1; j < n; ++j) a[i * n + j] = a[i * n + j] - coeff * a[k * n + j];
+        }
+    for (int i = 0; i < n; ++i) L[i * n + i] = 1.0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i <= j; ++i) U[i * n + j] = a[i * n + j];
+    for (int k = 0; k < n; ++k) {
+        b[k] = 1.0;
+        d[0] = b[0];
+        for (int i = 1; i < n; ++i) {
+            d[i] = b[i];
+            for (int j = 0; j <= i - 1; ++j) d[i] = d[i] - L[i * n + j] * d[j];
+        }
+        xx[n - 1] = d[n - 1] / U[(n - 1) * n + (n - 1)];
+        for (int i = n - 2; i >= 0; --i) {
+            xx[i] = d[i];
+            for (int j = n - 1; j >= i + 1; --j) xx[i] = xx[i] - U[i * n + j] * xx[j];
+            xx[i] = xx[i] / U[i * n + i];
+        }
+        for (int i = 0; i < n; ++i) c[i * n + k] = xx[i];
+        b[k] = 0.0;
+    }
+}
+
+/* mkVanderMondeMatrix: utils/interpol_mod.f90:584-601.  V[i*M+j] = Lhat_j(x_i). */
+void orc_vandermonde(int N, const double *x, double *V, double *Vinv) {
+    int M = N + 1;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            double dummy;
+            orc_legendre(j, x[i], &V[i * M + j], &dummy);
+        }
+    orc_invert(M, V, Vinv);
+}
+
+typedef struct {
+    int N, M, M2, M3;
+    double x[MAXM], w[MAXM];
+    double D[MAXM * MAXM];
+    double S[MAXM * MAXM];        /* split-form matrix 2D - W^-1 B (zero diagonal) */
+    double dgDiff[MAXM * MAXM];   /* weak-form volume matrix (reference scaling) */
+    double lagrM[MAXM], lagrP[MAXM];
+    double Vinv[MAXM * MAXM];
+} orc_ops;
+
+static void build_ops(int N, orc_ops *op) {
+    op->N = N;
+    op->M = N + 1;
+    op->M2 = op->M * op->M;
+    op->M3 = op->M2 * op->M;
+    int M = op->M;
+    orc_lgl_nodes_weights(N, op->x, op->w);
+    orc_diff_matrix(N, op->x, op->D);
+    for (int i = 0; i < M; ++i)
+        for (int m = 0; m < M; ++m) op->S[i * M + m] = (i == m) ? 0.0 : 2.0 * op->D[i * M + m];
+    /* kernel_operators_WeakDGDiffMatrix: kernel/dgsem/weakform/2d/kernel_operators_mod.f90:94-126:
+     * temp(i,j) = -2*w_j/w_i*D_ji ; mat = -mat/N_NODES */
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            double temp = -2.0 * op->w[j] / op->w[i] * op->D[j * M + i];
+            op->dgDiff[i * M + j] = -temp / (double)M;
+        }
+    /* kernel_operators_surfaceMatrix: kernel_operators_mod.f90:656-687; on LGL
+     * nodes the Lagrange polynomials at +-1 are Kronecker deltas
+     * (utils/lagrange_mod.f90:52-71, exact shortcut at the node). */
+    for (int i = 0; i < M; ++i) {
+        double lm = (i == 0) ? 1.0 : 0.0;
+        double lp = (i == M - 1) ? 1.0 : 0.0;
+        op->lagrM[i] = 2.0 / (double)M / op->w[i] * lm;
+        op->lagrP[i] = 2.0 / (double)M / op->w[i] * lp;
+    }
+    double V[MAXM * MAXM];
+    orc_vandermonde(N, op->x, V, op->Vinv);
+}
+
+/* ======================================================================== */
+/* Euler physics: equations/euler/equations_mod.f90                          */
+/* ======================================================================== */
+
+/* pressure_elemental: equations_mod.f90:131-140 */
+static inline double pres5(double kappa, const double *u) {
+    return (kappa - 1.0) * (u[ENER] - 0.5 / u[DENS] * (u[MOMX] * u[MOMX] + u[MOMY] * u[MOMY] + u[MOMZ] * u[MOMZ]));
+}
+
+double orc_pressure(const orc_cfg *c, const double *u) { return pres5(c->gamma, u); }
+
+/* xflux: equations_mod.f90:64-79 */
+static void xflux5(double kappa, const double *u, double *f) {
+    double pres = pres5(kappa, u);
+    double velx = u[MOMX] / u[DENS];
+    f[DENS] = u[MOMX];
+    f[MOMX] = velx * u[MOMX] + pres;
+    f[MOMY] = velx * u[MOMY];
+    f[MOMZ] = velx * u[MOMZ];
+    f[ENER] = velx * (u[ENER] + pres);
+}
+
+/* Rotations to the x direction and back: equations_mod.f90:81-127 (yflux,
+ * zflux), riemann/rusanov/riemann_mod.f90:41-101. */
+static void rot_in(int dir, const double *u, double *r) {
+    r[DENS] = u[DENS];
+    r[ENER] = u[ENER];
+    if (dir == 0) {
+        r[MOMX] = u[MOMX]; r[MOMY] = u[MOMY]; r[MOMZ] = u[MOMZ];
+    } else if (dir == 1) {
+        r[MOMX] = u[MOMY]; r[MOMY] = u[MOMZ]; r[MOMZ] = u[MOMX];
+    } else {
+        r[MOMX] = u[MOMZ]; r[MOMY] = u[MOMX]; r[MOMZ] = u[MOMY];
+    }
+}
+static void rot_out(int dir, const double *r, double *f) {
+    f[DENS] = r[DENS];
+    f[ENER] = r[ENER];
+    if (dir == 0) {
+        f[MOMX] = r[MOMX]; f[MOMY] = r[MOMY]; f[MOMZ] = r[MOMZ];
+    } else if (dir == 1) {
+        f[MOMX] = r[MOMZ]; f[MOMY] = r[MOMX]; f[MOMZ] = r[MOMY];
+    } else {
+        f[MOMX] = r[MOMY]; f[MOMY] = r[MOMZ]; f[MOMZ] = r[MOMX];
+    }
+}
+
+void orc_flux(const orc_cfg *c, int dir, const double *u, double *f) {
+    double ru[5], rf[5];
+    rot_in(dir, u, ru);
+    xflux5(c->gamma, ru, rf);
+    rot_out(dir, rf, f);
+}
+
+/* xlambdaMax_elemental + soundspeed_elemental: equations_mod.f90:288-297,384-393;
+ * y/z by permutation :436-443, :486-493. */
+static inline double xlmax5(double kappa, const double *u) {
+    return fabs(u[MOMX] / u[DENS]) + sqrt(kappa * pres5(kappa, u) / u[DENS]);
+}
+double orc_lambda_max(const orc_cfg *c, int dir, const double *u) {
+    double ru[5];
+    rot_in(dir, u, ru);
+    return xlmax5(c->gamma, ru);
+}
+
+/* isvalid_elemental: equations_mod.f90:337-344 */
+int orc_isvalid(const orc_cfg *c, const double *u) {
+    return (u[DENS] > 0.0) && (pres5(c->gamma, u) > 0.0);
+}
+
+/* cons2evec: equations_mod.f90:261-286 */
+void orc_cons2evec(const orc_cfg *c, const double *u, double *w) {
+    double kappa = c->gamma;
+    double rho = u[DENS];
+    double vx = u[MOMX] / rho, vy = u[MOMY] / rho, vz = u[MOMZ] / rho;
+    double p = pres5(kappa, u);
+    double s = log(p) - kappa * log(rho);
+    double b = rho / p;
+    w[DENS] = (kappa - s) / (kappa - 1.0) - 0.5 * b * (vx * vx + vy * vy + vz * vz);
+    w[MOMX] = b * vx;
+    w[MOMY] = b * vy;
+    w[MOMZ] = b * vz;
+    w[ENER] = -b;
+}
+
+/* cons2entr_elemental: equations_mod.f90:212-223 (mathematical entropy density) */
+double orc_entropy(const orc_cfg *c, const double *u) {
+    double kappa = c->gamma;
+    double s = log(pres5(kappa, u)) - kappa * log(u[DENS]);
+    return -u[DENS] * s / (kappa - 1.0);
+}
+
+/* ---- Riemann solvers ------------------------------------------------- */
+
+static void chandrashekar_x(double kappa, const double *u, const double *v, double *f);
+
+/* Rusanov: equations/euler/riemann/rusanov/riemann_mod.f90:7-24 */
+static void rusanov_x(double kappa, const double *uL, const double *uR, double *sf) {
+    double FL[5], FR[5];
+    xflux5(kappa, uL, FL);
+    xflux5(kappa, uR, FR);
+    double a = xlmax5(kappa, uL), b = xlmax5(kappa, uR);
+    double lmax = a > b ? a : b;
+    for (int h = 0; h < 5; ++h) sf[h] = 0.5 * (FL[h] + FR[h] + lmax * (uL[h] - uR[h]));
+}
+
+static inline double fmin3(double a, double b, double c) {
+    double m = a < b ? a : b;
+    return m < c ? m : c;
+}
+static inline double fmax3(double a, double b, double c) {
+    double m = a > b ? a : b;
+    return m > c ? m : c;
+}
+
+/* Hybrid HLLC/HLLE: equations/euler/riemann/hllc/riemann_mod.f90:7-104 */
+static void hllc_x(double kappa, const double *U_L, const double *U_R, double *F) {
+    double F_L[5], F_R[5], F_HLLE[5], UStar[5];
+    const double beta = sqrt(0.5 * (kappa - 1) / kappa);
+    double V_L = U_L[MOMX] / U_L[DENS];
+    double V_R = U_R[MOMX] / U_R[DENS];
+    double p_L = pres5(kappa, U_L);
+    double p_R = pres5(kappa, U_R);
+    double c_L = sqrt(kappa * p_L / U_L[DENS]);
+    double c_R = sqrt(kappa * p_R / U_R[DENS]);
+    double H_L = (U_L[ENER] + p_L) / U_L[DENS];
+    double H_R = (U_R[ENER] + p_R) / U_R[DENS];
+    double sL = sqrt(U_L[DENS]), sR = sqrt(U_R[DENS]);
+    double HTilde = (sL * H_L + sR * H_R) / (sL + sR);
+    double vTilde = (sL * V_L + sR * V_R) / (sL + sR);
+    double aTilde = sqrt((kappa - 1.0) * (HTilde - 0.5 * (vTilde * vTilde)));
+    double S_L = fmin3(vTilde - aTilde, V_L - beta * c_L, 0.0);
+    double S_R = fmax3(vTilde + aTilde, V_R + beta * c_R, 0.0);
+    if (S_L >= 0.0) {
+        xflux5(kappa, U_L, F);
+    } else if (S_R <= 0.0) {
+        xflux5(kappa, U_R, F);
+    } else {
+        xflux5(kappa, U_L, F_L);
+        xflux5(kappa, U_R, F_R);
+        double smooth = sqrt(fabs(p_L - p_R) / (p_L + p_R));
+        for (int h = 0; h < 5; ++h)
+            F_HLLE[h] = (S_R * F_L[h] - S_L * F_R[h] + S_L * S_R * (U_R[h] - U_L[h])) / (S_R - S_L);
+        double SStar = (p_R - p_L + U_L[MOMX] * (S_L - V_L) - U_R[MOMX] * (S_R - V_R)) /
+                       (U_L[DENS] * (S_L - V_L) - U_R[DENS] * (S_R - V_R));
+        if (S_L <= 0.0 && 0.0 <= SStar) {
+            double QStar = U_L[DENS] * (S_L - V_L) / (S_L - SStar);
+            UStar[DENS] = QStar;
+            UStar[MOMX] = QStar * SStar;
+            UStar[MOMY] = QStar * U_L[MOMY] / U_L[DENS];
+            UStar[MOMZ] = QStar * U_L[MOMZ] / U_L[DENS];
+            UStar[ENER] = QStar * (U_L[ENER] / U_L[DENS] + (SStar - V_L) * (SStar + p_L / U_L[DENS] / (S_L - V_L)));
+            for (int h = 0; h < 5; ++h)
+                F[h] = smooth * F_HLLE[h] + (1.0 - smooth) * (F_L[h] + S_L * (UStar[h] - U_L[h]));
+        } else {
+            double QStar = U_R[DENS] * (S_R - V_R) / (S_R - SStar);
+            UStar[DENS] = QStar;
+            UStar[MOMX] = QStar * SStar;
+            UStar[MOMY] = QStar * U_R[MOMY] / U_R[DENS];
+            UStar[MOMZ] = QStar * U_R[MOMZ] / U_R[DENS];
+            UStar[ENER] = QStar * (U_R[ENER] / U_R[DENS] + (SStar - V_R) * (SStar + p_R / U_R[DENS] / (S_R - V_R)));
+            for (int h = 0; h < 5; ++h)
+                F[h] = smooth * F_HLLE[h] + (1.0 - smooth) * (F_R[h] + S_R * (UStar[h] - U_R[h]));
+        }
+    }
+}
+
+void orc_riemann(const orc_cfg *c, int kind, int dir, const double *uL, const double *uR, double *f) {
+    double ruL[5], ruR[5], rf[5];
+    rot_in(dir, uL, ruL);
+    rot_in(dir, uR, ruR);
+    if (kind == ORC_SURF_HLLC)
+        hllc_x(c->gamma, ruL, ruR, rf);
+    else if (kind == ORC_SURF_EC_TEST)
+        chandrashekar_x(c->gamma, ruL, ruR, rf);
+    else
+        rusanov_x(c->gamma, ruL, ruR, rf);
+    rot_out(dir, rf, f);
+}
+
+/* ---- two-point volume fluxes ----------------------------------------- */
+
+/* logmean: equations/euler/two_point/chandrashekar/two_point_flux_mod.f90:13-32.
+ * The series branch is restated as written (its u^2 coefficient carries the
+ * reference's sign). */
+double orc_logmean(double a, double b) {
+    const double eps = 1e-4;
+    const double c1 = 1.0 / 6.0, c2 = 2.0 / 45.0, c3 = 22.0 / 945.0;
+    double x = a / b;
+    if (fabs(a - b) < eps) {
+        double u = (x * (x - 2.0) + 1.0) / (x * (x + 2.0) + 1.0);
+        return (a + b) * (0.5 - u * (c1 - u * (c2 - c3 * u)));
+    }
+    return (a - b) / log(x);
+}
+
+/* Chandrashekar xflux: two_point_flux_mod.f90:34-77 */
+static void chandrashekar_x(double kappa, const double *u, const double *v, double *f) {
+    double pu = pres5(kappa, u);
+    double pv = pres5(kappa, v);
+    double bu = 0.5 * u[DENS] / pu;
+    double bv = 0.5 * v[DENS] / pv;
+    double uu0 = u[MOMX] / u[DENS], uu1 = u[MOMY] / u[DENS], uu2 = u[MOMZ] / u[DENS];
+    double vv0 = v[MOMX] / v[DENS], vv1 = v[MOMY] / v[DENS], vv2 = v[MOMZ] / v[DENS];
+    double dens = orc_logmean(u[DENS], v[DENS]);
+    double velx = 0.5 * (uu0 + vv0);
+    double vely = 0.5 * (uu1 + vv1);
+    double velz = 0.5 * (uu2 + vv2);
+    double pres = 0.5 * (u[DENS] + v[DENS]) / (bu + bv);
+    /* sum(uu*uu + vv*vv) over the three components, left to right */
+    double ssum = (uu0 * uu0 + vv0 * vv0) + (uu1 * uu1 + vv1 * vv1) + (uu2 * uu2 + vv2 * vv2);
+    double enth = 0.5 / (kappa - 1.0) / orc_logmean(bu, bv) + pres / dens + velx * velx + vely * vely +
+                  velz * velz - 0.25 * ssum;
+    f[DENS] = dens * velx;
+    f[MOMX] = f[DENS] * velx + pres;
+    f[MOMY] = f[DENS] * vely;
+    f[MOMZ] = f[DENS] * velz;
+    f[ENER] = f[DENS] * enth;
+}
+
+/* central: equations/euler/two_point/standard/two_point_flux_mod.f90:13-23 */
+static void central_x(double kappa, const double *u, const double *v, double *r) {
+    double fu[5], fv[5];
+    xflux5(kappa, u, fu);
+    xflux5(kappa, v, fv);
+    for (int h = 0; h < 5; ++h) r[h] = 0.5 * (fu[h] + fv[h]);
+}
+
+void orc_two_point(const orc_cfg *c, int kind, int dir, const double *u, const double *v, double *f) {
+    double ru[5], rv[5], rf[5];
+    rot_in(dir, u, ru);
+    rot_in(dir, v, rv);
+    if (kind == ORC_VOL_EC)
+        chandrashekar_x(c->gamma, ru, rv, rf);
+    else
+        central_x(c->gamma, ru, rv, rf);
+    rot_out(dir, rf, f);
+}
+
+/* ======================================================================== */
+/* Element operators                                                        */
+/* ======================================================================== */
+
+static inline int node_index(int M, int d, int l, int a, int b) {
+    /* node on the line along direction d at position l, transverse (a,b) */
+    if (d == 0) return l + M * a + M * M * b;
+    if (d == 1) return a + M * l + M * M * b;
+    return a + M * b + M * M * l;
+}
+
+/* Persson-Peraire modal-energy indicator on rho*p, as used for DG/FV
+ * blending by Hennemann et al. (JCP 426, 2021, sec. 3).  The modal basis is
+ * the reference's normalised Legendre Vandermonde (utils/interpol_mod.f90:
+ * 274-311, 584-601).  No reference implementation exists in the snapshot
+ * (the blended kernels are missing): parity is pinned by closed forms. */
+static double indicator_element(const orc_cfg *c, const orc_ops *op, const double *ue, double *energy_out) {
+    int M = op->M, M3 = op->M3;
+    double q[MAXM * MAXM * MAXM], t1[MAXM * MAXM * MAXM], t2[MAXM * MAXM * MAXM];
+    double uu[5];
+    for (int n = 0; n < M3; ++n) {
+        for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
+        q[n] = uu[DENS] * pres5(c->gamma, uu);
+    }
+    /* modal = (Vinv x Vinv x Vinv) q, sum-factorised x, then y, then z */
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < M; ++j)
+            for (int a = 0; a < M; ++a) {
+                double s = 0.0;
+                for (int i = 0; i < M; ++i) s += op->Vinv[a * M + i] * q[i + M * j + M * M * k];
+                t1[a + M * j + M * M * k] = s;
+            }
+    for (int k = 0; k < M; ++k)
+        for (int b = 0; b < M; ++b)
+            for (int a = 0; a < M; ++a) {
+                double s = 0.0;
+                for (int j = 0; j < M; ++j) s += op->Vinv[b * M + j] * t1[a + M * j + M * M * k];
+                t2[a + M * b + M * M * k] = s;
+            }
+    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
+    for (int cc = 0; cc < M; ++cc)
+        for (int b = 0; b < M; ++b)
+            for (int a = 0; a < M; ++a) {
+                double s = 0.0;
+                for (int k = 0; k < M; ++k) s += op->Vinv[cc * M + k] * t2[a + M * b + M * M * k];
+                double e2 = s * s;
+                total += e2;
+                if (a < M - 1 && b < M - 1 && cc < M - 1) clip1 += e2;
+                if (a < M - 2 && b < M - 2 && cc < M - 2) clip2 += e2;
+            }
+    double energy = 0.0;
+    if (total > 0.0) {
+        double f1 = (total - clip1) / total;
+        double f2 = (total - clip2) / total;
+        energy = f1 > f2 ? f1 : f2;
+    }
+    if (energy_out) *energy_out = energy;
+    double threshold = 0.5 * pow(10.0, -1.8 * pow((double)M, 0.25));
+    double s = log((1.0 - 0.0001) / 0.0001);
+    double alpha = 1.0 / (1.0 + exp(-s / threshold * (energy - threshold)));
+    if (alpha < c->alpha_min)
+        alpha = 0.0;
+    else if (alpha > 1.0 - c->alpha_min)
+        alpha = 1.0;
+    if (alpha > c->alpha_max) alpha = c->alpha_max;
+    return alpha;
+}
+
+void orc_indicator(const orc_cfg *c, int64_t nelem, const double *u, double *alpha, double *energy) {
+    orc_ops op;
+    build_ops(c->N, &op);
+    int V = orc_nvars(c);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < nelem; ++e) {
+        double en;
+        double a = indicator_element(c, &op, u + e * V * op.M3, &en);
+        if (alpha) alpha[e] = a;
+        if (energy) energy[e] = en;
+    }
+}
+
+/* One element's RHS.
+ *  weak form:  3-D LGL extension of kernel_calc_flux,
+ *              kernel/dgsem/weakform/2d/kernel_flux_mod.f90:197-277
+ *              (rhs = surface + volume, per direction, sdx = N_NODES/dx).
+ *  flux differencing (split form, Gassner/Winters/Kopriva 2016):
+ *              rhs_i = -(2/dx) [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ]
+ *  FV subcells on the LGL sub-grid (Hennemann et al. 2021), first order, with
+ *              the surface Riemann solver at the interior interfaces; the
+ *              element-face fluxes are the DG surface fluxes:
+ *              rhs_i = -(2/dx) (F_{i+1/2} - F_{i-1/2}) / w_i
+ *  blended:    rhs = (1-alpha) rhs_DG + alpha rhs_FV. */
+static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const int32_t *nbr,
+                        const double *delta, const double *u, double alpha, double *out) {
+    const int M = op->M, M3 = op->M3, V = 5;
+    const double *ue = u + e * V * M3;
+    double sfl[3][5][MAXM * MAXM * MAXM];
+    double vfl[3][5][MAXM * MAXM * MAXM];
+    double fvr[3][5][MAXM * MAXM * MAXM];
+    int do_fv = (c->indicator != ORC_IND_NONE) && alpha > 0.0;
+
+    for (int d = 0; d < 3; ++d) {
+        const int64_t em = nbr[e * 6 + 2 * d], ep = nbr[e * 6 + 2 * d + 1];
+        const double dx = delta[e * 3 + d];
+        const double sdx = (double)M / dx;
+        const double jac = 2.0 / dx;
+        for (int b = 0; b < M; ++b)
+            for (int a = 0; a < M; ++a) {
+                double ul[MAXM][5], unm[5], unp[5], fm[5], fp[5];
+                for (int l = 0; l < M; ++l)
+                    for (int h = 0; h < V; ++h) ul[l][h] = ue[h * M3 + node_index(M, d, l, a, b)];
+                for (int h = 0; h < V; ++h) {
+                    unm[h] = u[em * V * M3 + h * M3 + node_index(M, d, M - 1, a, b)];
+                    unp[h] = u[ep * V * M3 + h * M3 + node_index(M, d, 0, a, b)];
+                }
+                orc_riemann(c, c->surface, d, unm, ul[0], fm);
+                orc_riemann(c, c->surface, d, ul[M - 1], unp, fp);
+
+                if (c->volume == ORC_VOL_WEAK) {
+                    double fx[MAXM][5];
+                    for (int l = 0; l < M; ++l) {
+                        orc_flux(c, d, ul[l], fx[l]);
+                        for (int h = 0; h < V; ++h) fx[l][h] = sdx * fx[l][h];
+                    }
+                    for (int i = 0; i < M; ++i) {
+                        int n = node_index(M, d, i, a, b);
+                        for (int h = 0; h < V; ++h) {
+                            sfl[d][h][n] = op->lagrM[i] * (sdx * fm[h]) + (-op->lagrP[i]) * (sdx * fp[h]);
+                            double s = 0.0;
+                            for (int j = 0; j < M; ++j) s += op->dgDiff[i * M + j] * fx[j][h];
+                            vfl[d][h][n] = s;
+                        }
+                    }
+                } else {
+                    double acc[MAXM][5];
+                    for (int i = 0; i < M; ++i)
+                        for (int h = 0; h < V; ++h) acc[i][h] = 0.0;
+                    for (int i = 0; i < M; ++i)
+                        for (int m = 0; m < M; ++m) {
+                            if (m == i) continue;
+                            double F[5];
+                            orc_two_point(c, c->volume, d, ul[i], ul[m], F);
+                            for (int h = 0; h < V; ++h) acc[i][h] += op->S[i * M + m] * F[h];
+                        }
+                    for (int i = 0; i < M; ++i) {
+                        int n = node_index(M, d, i, a, b);
+                        for (int h = 0; h < V; ++h) {
+                            double surf = 0.0;
+                            if (i == M - 1) surf += fp[h];
+                            if (i == 0) surf -= fm[h];
+                            vfl[d][h][n] = -jac * (acc[i][h] + surf / op->w[i]);
+                            sfl[d][h][n] = 0.0;
+                        }
+                    }
+                }
+                if (do_fv) {
+                    double Fi[MAXM + 1][5];
+                    for (int h = 0; h < V; ++h) {
+                        Fi[0][h] = fm[h];
+                        Fi[M][h] = fp[h];
+                    }
+                    for (int l = 0; l < M - 1; ++l) orc_riemann(c, c->surface, d, ul[l], ul[l + 1], Fi[l + 1]);
+                    for (int i = 0; i < M; ++i) {
+                        int n = node_index(M, d, i, a, b);
+                        for (int h = 0; h < V; ++h) fvr[d][h][n] = -jac * (Fi[i + 1][h] - Fi[i][h]) / op->w[i];
+                    }
+                }
+            }
+    }
+    for (int h = 0; h < V; ++h)
+        for (int n = 0; n < M3; ++n) {
+            double dg = (sfl[0][h][n] + sfl[1][h][n] + sfl[2][h][n]) + (vfl[0][h][n] + vfl[1][h][n] + vfl[2][h][n]);
+            if (do_fv) {
+                double fv = fvr[0][h][n] + fvr[1][h][n] + fvr[2][h][n];
+                out[h * M3 + n] = (1.0 - alpha) * dg + alpha * fv;
+            } else {
+                out[h * M3 + n] = dg;
+            }
+        }
+}
+
+void orc_rhs(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, const double *u,
+             const double *alpha, double *rhs) {
+    orc_ops op;
+    build_ops(c->N, &op);
+    const int V = orc_nvars(c);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < nelem; ++e) {
+        double a = 0.0;
+        if (c->indicator != ORC_IND_NONE) a = alpha ? alpha[e] : indicator_element(c, &op, u + e * V * op.M3, NULL);
+        rhs_element(c, &op, e, nbr, delta, u, a, rhs + e * V * op.M3);
+    }
+}
+
+/* CFL rule: kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-121 with the
+ * 3-D per-direction minimum of kernel/mcfv/3d/kernel_timestep_mod.f90:60-106:
+ * dt = 0.5*CFL/(2*N_NODES-1) * min_e min_d delta_d / max_nodes lambda_d. */
+double orc_calc_timestep(const orc_cfg *c, int64_t nelem, const double *delta, const double *u, double *lmax_out,
+                         double *vmax_out) {
+    const int M = c->N + 1, M3 = M * M * M, V = orc_nvars(c);
+    double dtmin = 1.7976931348623157e308, lmax_g = 0.0, vmax_g = 0.0;
+#pragma omp parallel for schedule(static) reduction(min : dtmin) reduction(max : lmax_g, vmax_g)
+    for (int64_t e = 0; e < nelem; ++e) {
+        const double *ue = u + e * V * M3;
+        double lm[3] = {0.0, 0.0, 0.0}, vm = 0.0;
+        for (int n = 0; n < M3; ++n) {
+            double uu[5];
+            for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
+            for (int d = 0; d < 3; ++d) {
+                double l = orc_lambda_max(c, d, uu);
+                if (l > lm[d]) lm[d] = l;
+                double v = fabs(uu[1 + d] / uu[DENS]);
+                if (v > vm) vm = v;
+            }
+        }
+        double dt = 1.7976931348623157e308;
+        double t0 = delta[e * 3 + 0] / lm[0], t1 = delta[e * 3 + 1] / lm[1], t2 = delta[e * 3 + 2] / lm[2];
+        double tm = fmin3(t0, t1, t2);
+        dt = dt < tm ? dt : tm;
+        if (dt < dtmin) dtmin = dt;
+        double l3 = fmax3(lm[0], lm[1], lm[2]);
+        if (l3 > lmax_g) lmax_g = l3;
+        if (vm > vmax_g) vmax_g = vm;
+    }
+    if (lmax_out) *lmax_out = lmax_g;
+    if (vmax_out) *vmax_out = vmax_g;
+    return 0.5 * c->cfl / (double)(2 * M - 1) * dtmin;
+}
+
+/* Positivity repair: hierarchical 2x2x2 tile squeeze,
+ * kernel/mcfv/3d/kernel_validate_mod.f90:159-274 (repair_decr = 0.01,
+ * kernel/mcfv/3d/kernel_config_mod.f90:5).  On LGL nodes the tile and group
+ * means are quadrature-weighted so that the squeeze keeps the element
+ * integral; with equal weights this reduces to the reference's
+ * ws = 1/nn^3 tiles and 0.125*sum(tiles) means.  Needs N_NODES to be a power
+ * of two (kernel_validate_mod.f90:182); for other N a single squeeze of the
+ * whole element towards its mean is used (the reference's disabled
+ * alternative, kernel_validate_mod.f90:191-210, with repair_decr steps).
+ * Returns 1 if the element is valid afterwards. */
+static int elem_valid(double kappa, const double *ue, int M3) {
+    for (int n = 0; n < M3; ++n) {
+        double uu[5];
+        for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
+        if (!((uu[DENS] > 0.0) && (pres5(kappa, uu) > 0.0))) return 0;
+    }
+    return 1;
+}
+
+static int repair_element(const orc_cfg *c, const orc_ops *op, double *ue) {
+    const int M = op->M, M3 = op->M3;
+    const double kappa = c->gamma;
+    const double repair_decr = 0.01;
+    if (elem_valid(kappa, ue, M3)) return 1;
+    int nlevels = 0;
+    while ((1 << nlevels) < M) ++nlevels;
+    int pow2 = ((1 << nlevels) == M);
+    if (!pow2) {
+        double means[5] = {0, 0, 0, 0, 0}, wsum = 0.0;
+        for (int k = 0; k < M; ++k)
+            for (int j = 0; j < M; ++j)
+                for (int i = 0; i < M; ++i) {
+                    double ww = op->w[i] * op->w[j] * op->w[k];
+                    wsum += ww;
+                    for (int h = 0; h < 5; ++h) means[h] += ww * ue[h * M3 + i + M * j + M * M * k];
+                }
+        for (int h = 0; h < 5; ++h) means[h] = means[h] / wsum;
+        double squeeze = 1.0;
+        double orig[5 * MAXM * MAXM * MAXM];
+        memcpy(orig, ue, sizeof(double) * 5 * M3);
+        while (squeeze > 0.0) {
+            squeeze = squeeze - repair_decr;
+            if (squeeze < 0.0) squeeze = 0.0;
+            for (int h = 0; h < 5; ++h)
+                for (int n = 0; n < M3; ++n) ue[h * M3 + n] = means[h] + squeeze * (orig[h * M3 + n] - means[h]);
+            if (elem_valid(kappa, ue, M3)) break;
+        }
+        return elem_valid(kappa, ue, M3);
+    }
+    for (int level = 1; level <= nlevels; ++level) {
+        int mm = 1 << (level - 1);
+        int nn = 1 << (nlevels - level);
+        for (int r = 0; r < mm; ++r)
+            for (int p = 0; p < mm; ++p)
+                for (int q = 0; q < mm; ++q) {
+                    double tiles[8][5], tw[8];
+                    for (int t = 0; t < 8; ++t) {
+                        tw[t] = 0.0;
+                        for (int h = 0; h < 5; ++h) tiles[t][h] = 0.0;
+                    }
+                    for (int kk = 0; kk < 2; ++kk)
+                        for (int jj = 0; jj < 2; ++jj)
+                            for (int ii = 0; ii < 2; ++ii) {
+                                int t = ii + 2 * jj + 4 * kk;
+                                for (int k = (2 * r + kk) * nn; k < (2 * r + kk + 1) * nn; ++k)
+                                    for (int j = (2 * p + jj) * nn; j < (2 * p + jj + 1) * nn; ++j)
+                                        for (int i = (2 * q + ii) * nn; i < (2 * q + ii + 1) * nn; ++i) {
+                                            double ww = op->w[i] * op->w[j] * op->w[k];
+                                            tw[t] += ww;
+                                            for (int h = 0; h < 5; ++h)
+                                                tiles[t][h] += ww * ue[h * M3 + i + M * j + M * M * k];
+                                        }
+                                for (int h = 0; h < 5; ++h) tiles[t][h] = tiles[t][h] / tw[t];
+                            }
+                    int ok = 1;
+                    for (int t = 0; t < 8; ++t)
+                        if (!((tiles[t][DENS] > 0.0) && (pres5(kappa, tiles[t]) > 0.0))) ok = 0;
+                    if (ok) continue;
+                    double means[5], twsum = 0.0;
+                    for (int t = 0; t < 8; ++t) twsum += tw[t];
+                    for (int h = 0; h < 5; ++h) {
+                        double s = 0.0;
+                        for (int t = 0; t < 8; ++t) s += tw[t] * tiles[t][h];
+                        means[h] = s / twsum;
+                    }
+                    double hiles[8][5];
+                    for (int t = 0; t < 8; ++t)
+                        for (int h = 0; h < 5; ++h) hiles[t][h] = tiles[t][h] - means[h];
+                    double squeeze = 1.0;
+                    while (squeeze > 0.0) {
+                        squeeze = squeeze - repair_decr;
+                        if (squeeze < 0.0) squeeze = 0.0;
+                        int okt = 1;
+                        for (int t = 0; t < 8; ++t) {
+                            double tt[5];
+                            for (int h = 0; h < 5; ++h) tt[h] = means[h] + squeeze * hiles[t][h];
+                            if (!((tt[DENS] > 0.0) && (pres5(kappa, tt) > 0.0))) okt = 0;
+                        }
+                        if (okt) break;
+                    }
+                    for (int k = 2 * r * nn; k < (2 * r + 2) * nn; ++k)
+                        for (int j = 2 * p * nn; j < (2 * p + 2) * nn; ++j)
+                            for (int i = 2 * q * nn; i < (2 * q + 2) * nn; ++i)
+                                for (int h = 0; h < 5; ++h) {
+                                    double *x = &ue[h * M3 + i + M * j + M * M * k];
+                                    *x = means[h] + squeeze * (*x - means[h]);
+                                }
+                }
+    }
+    return elem_valid(kappa, ue, M3);
+}
+
+int orc_repair(const orc_cfg *c, double *ue) {
+    orc_ops op;
+    build_ops(c->N, &op);
+    return repair_element(c, &op, ue);
+}
+
+/* ---- time integration --------------------------------------------------- */
+
+/* low-storage/4-5/timedisc_const_mod.f90:5-29 */
+void orc_lsrk54_coeffs(double *a, double *b, double *cc) {
+    a[0] = 0.0;
+    a[1] = -567301805773.0 / 1357537059087.0;
+    a[2] = -2404267990393.0 / 2016746695238.0;
+    a[3] = -3550918686646.0 / 2091501179385.0;
+    a[4] = -1275806237668.0 / 842570457699.0;
+    b[0] = 0.0;
+    b[1] = 1432997174477.0 / 9575080441755.0;
+    b[2] = 2526269341429.0 / 6820363962896.0;
+    b[3] = 2006345519317.0 / 3224310063776.0;
+    b[4] = 2802321613138.0 / 2924317926251.0;
+    cc[0] = 1432997174477.0 / 9575080441755.0;
+    cc[1] = 5161836677717.0 / 13612068292357.0;
+    cc[2] = 1720146321549.0 / 2090206949498.0;
+    cc[3] = 3134564353537.0 / 4481467310338.0;
+    cc[4] = 2277821191437.0 / 14882151754819.0;
+}
+
+/* ssp/4-5/timedisc_const_mod.f90:5-32, order A0..A3, B0..B3, C0..C3, D0..D3,
+ * E0..E6 */
+void orc_ssprk54_coeffs(double *k) {
+    static const double v[23] = {
+        1.00000000000000, 1.00000000000000, 0.00000000000000, 0.39175222700392,
+        0.39175222700392, 0.44437049406734, 0.55562950593266, 0.36841059262959,
+        0.58607968896780, 0.62010185138540, 0.37989814861460, 0.25189177424738,
+        0.47454236302687, 0.17807995410773, 0.82192004589227, 0.54497475021237,
+        0.93501063100924, 0.00683325884039, 0.34833675773694, 0.22600748319395,
+        0.51723167208978, 0.12759831133288, 0.08460416338212};
+    memcpy(k, v, sizeof(v));
+}
+
+/* One RK stage s (1..5) and one time step.  LSRK 5-4: timedisc_probe_mod.f90:31-37 and
+ * timedisc_hooks_mod.f90:39-47 (low-storage/4-5); SSPRK(5,4):
+ * ssp/4-5/timedisc_probe_mod.f90:24-46, timedisc_hooks_mod.f90:40-67.
+ * Per stage: RHS of the stage's input state for every element (faces before
+ * volume, kernel_hooks two-phase order), probe, repair, register update. */
+int orc_stage(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, double *u, double *r0,
+              double *r1, int s, double dt) {
+    orc_ops op;
+    build_ops(c->N, &op);
+    const int V = orc_nvars(c);
+    const int64_t per = (int64_t)V * op.M3;
+    double *rhs = (double *)malloc(sizeof(double) * per * nelem);
+    double *alpha = (double *)malloc(sizeof(double) * nelem);
+    double la[5], lb[5], lc[5], sk[23];
+    orc_lsrk54_coeffs(la, lb, lc);
+    orc_ssprk54_coeffs(sk);
+    const double A3 = sk[3], B1 = sk[5], B2 = sk[6], B3 = sk[7], C1 = sk[9], C2 = sk[10], C3 = sk[11];
+    const double D1 = sk[13], D2 = sk[14], D3 = sk[15], E1 = sk[17], E2 = sk[18], E3 = sk[19];
+    const double E4 = sk[20], E5 = sk[21], E6 = sk[22];
+    int fail = 0;
+    if (c->indicator != ORC_IND_NONE) orc_indicator(c, nelem, u, alpha, NULL);
+    orc_rhs(c, nelem, nbr, delta, u, c->indicator != ORC_IND_NONE ? alpha : NULL, rhs);
+#pragma omp parallel for schedule(static) reduction(| : fail)
+    for (int64_t e = 0; e < nelem; ++e) {
+        double *ue = u + e * per, *fe = rhs + e * per, *q0 = r0 + e * per, *q1 = r1 ? r1 + e * per : NULL;
+        double st[MAXV * MAXM * MAXM * MAXM];
+        if (c->timedisc == ORC_TD_LSRK54) {
+            for (int64_t n = 0; n < per; ++n) {
+                double reg = (s == 1) ? fe[n] : la[s - 1] * q0[n] + fe[n];
+                st[n] = ue[n] + lc[s - 1] * dt * reg;
+                q0[n] = reg;
+            }
+        } else {
+            for (int64_t n = 0; n < per; ++n) {
+                switch (s) {
+                case 1: st[n] = ue[n] + A3 * dt * fe[n]; break;
+                case 2: st[n] = B1 * q0[n] + B2 * ue[n] + B3 * dt * fe[n]; break;
+                case 3: st[n] = C1 * q0[n] + C2 * ue[n] + C3 * dt * fe[n]; break;
+                case 4: st[n] = D1 * q0[n] + D2 * ue[n] + D3 * dt * fe[n]; break;
+                default: st[n] = E1 * q0[n] + E2 * ue[n] + E3 * dt * fe[n] + q1[n]; break;
+                }
+            }
+        }
+        if (c->repair && !repair_element(c, &op, st)) fail |= 1;
+        if (c->timedisc == ORC_TD_SSPRK54) {
+            for (int64_t n = 0; n < per; ++n) {
+                switch (s) {
+                case 1: q0[n] = ue[n]; break;
+                case 2: q1[n] = E4 * st[n]; break;
+                case 3: q1[n] = q1[n] + E5 * st[n]; break;
+                case 4: q1[n] = q1[n] + E6 * dt * fe[n]; break;
+                default: break;
+                }
+            }
+        }
+        memcpy(ue, st, sizeof(double) * per);
+    }
+    free(rhs);
+    free(alpha);
+    return fail;
+}
+
+int orc_step(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, double *u, double *r0,
+             double *r1, double dt) {
+    for (int s = 1; s <= 5; ++s) {
+        int rc = orc_stage(c, nelem, nbr, delta, u, r0, r1, s, dt);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int orc_run(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, double *u, int nsteps,
+            double *dt_hist) {
+    const int V = orc_nvars(c);
+    const int M = c->N + 1;
+    const int64_t total = (int64_t)V * M * M * M * nelem;
+    double *r0 = (double *)calloc(total, sizeof(double));
+    double *r1 = (double *)calloc(total, sizeof(double));
+    int rc = 0;
+    for (int it = 0; it < nsteps; ++it) {
+        double dt = orc_calc_timestep(c, nelem, delta, u, NULL, NULL);
+        if (dt_hist) dt_hist[it] = dt;
+        rc = orc_step(c, nelem, nbr, delta, u, r0, r1, dt);
+        if (rc) break;
+    }
+    free(r0);
+    free(r1);
+    return rc;
+}

@@ -1,0 +1,372 @@
+"""ctypes view of the FP64 CPU oracle (oracle/libnemo_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module; the product
+package ``nemo_amd`` never does.  It is the checker, never the thing
+measured or shipped.
+
+Arrays follow the product's C-ABI layout: ``u[elem][var][node]`` with
+``node = i + M*j + M*M*k`` (x fastest, ``M = N+1``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libnemo_oracle.so")
+_REF_A = os.path.join(_HERE, "_ref", "libnemo_ref_a.so")
+_REF_B = os.path.join(_HERE, "_ref", "libnemo_ref_b.so")
+
+EQ_EULER, EQ_MHD = 0, 1
+VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
+SURF_RUSANOV, SURF_HLLC, SURF_HLLD, SURF_EC_TEST = 0, 1, 2, 3
+TD_LSRK54, TD_SSPRK54 = 0, 1
+IND_NONE, IND_PP = 0, 1
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32),
+        ("eq", ctypes.c_int32),
+        ("volume", ctypes.c_int32),
+        ("surface", ctypes.c_int32),
+        ("indicator", ctypes.c_int32),
+        ("timedisc", ctypes.c_int32),
+        ("repair", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("alpha_max", ctypes.c_double),
+        ("alpha_min", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("mu0", ctypes.c_double),
+        ("glm_ch", ctypes.c_double),
+        ("glm_alpha", ctypes.c_double),
+    ]
+
+
+def make_cfg(N=3, eq=EQ_EULER, volume=VOL_WEAK, surface=SURF_RUSANOV,
+             indicator=IND_NONE, timedisc=TD_SSPRK54, repair=0, gamma=1.4,
+             alpha_max=0.5, alpha_min=0.001, cfl=0.5, mu0=1.0, glm_ch=0.0,
+             glm_alpha=0.0) -> Cfg:
+    return Cfg(N, eq, volume, surface, indicator, timedisc, repair, 0, gamma,
+               alpha_max, alpha_min, cfl, mu0, glm_ch, glm_alpha)
+
+
+_P = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_PI)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError(f"oracle library not built: {_LIB} (run make -C oracle)")
+        L = ctypes.CDLL(_LIB)
+        cp = ctypes.POINTER(Cfg)
+        L.orc_pressure.restype = ctypes.c_double
+        L.orc_lambda_max.restype = ctypes.c_double
+        L.orc_logmean.restype = ctypes.c_double
+        L.orc_logmean.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_entropy.restype = ctypes.c_double
+        L.orc_calc_timestep.restype = ctypes.c_double
+        L.orc_calc_timestep.argtypes = [cp, ctypes.c_int64, _P, _P, _P, _P]
+        L.orc_rhs.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, _P, _P]
+        L.orc_indicator.argtypes = [cp, ctypes.c_int64, _P, _P, _P]
+        L.orc_step.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, _P, _P, ctypes.c_double]
+        L.orc_stage.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, _P, _P, ctypes.c_int, ctypes.c_double]
+        L.orc_run.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, ctypes.c_int, _P]
+        L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_double, _P, _P]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+# ---- operators -----------------------------------------------------------
+
+def lgl(N: int):
+    x = np.zeros(N + 1)
+    w = np.zeros(N + 1)
+    lib().orc_lgl_nodes_weights(ctypes.c_int(N), _ptr(x), _ptr(w))
+    return x, w
+
+
+def diff_matrix(N: int, x: np.ndarray) -> np.ndarray:
+    D = np.zeros((N + 1, N + 1))
+    lib().orc_diff_matrix(ctypes.c_int(N), _ptr(np.ascontiguousarray(x)), _ptr(D))
+    return D
+
+
+def vandermonde(N: int, x: np.ndarray):
+    V = np.zeros((N + 1, N + 1))
+    Vi = np.zeros((N + 1, N + 1))
+    lib().orc_vandermonde(ctypes.c_int(N), _ptr(np.ascontiguousarray(x)), _ptr(V), _ptr(Vi))
+    return V, Vi
+
+
+def legendre(n: int, x: float):
+    L = np.zeros(1)
+    dL = np.zeros(1)
+    lib().orc_legendre(n, x, _ptr(L), _ptr(dL))
+    return float(L[0]), float(dL[0])
+
+
+# ---- pointwise -----------------------------------------------------------
+
+def pressure(cfg: Cfg, u) -> float:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_pressure(ctypes.byref(cfg), _ptr(u))
+
+
+def flux(cfg: Cfg, d: int, u) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.zeros_like(u)
+    lib().orc_flux(ctypes.byref(cfg), ctypes.c_int(d), _ptr(u), _ptr(f))
+    return f
+
+
+def lambda_max(cfg: Cfg, d: int, u) -> float:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_lambda_max(ctypes.byref(cfg), ctypes.c_int(d), _ptr(u))
+
+
+def riemann(cfg: Cfg, kind: int, d: int, uL, uR) -> np.ndarray:
+    uL = np.ascontiguousarray(uL, dtype=np.float64)
+    uR = np.ascontiguousarray(uR, dtype=np.float64)
+    f = np.zeros_like(uL)
+    lib().orc_riemann(ctypes.byref(cfg), ctypes.c_int(kind), ctypes.c_int(d), _ptr(uL), _ptr(uR), _ptr(f))
+    return f
+
+
+def two_point(cfg: Cfg, kind: int, d: int, u, v) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    f = np.zeros_like(u)
+    lib().orc_two_point(ctypes.byref(cfg), ctypes.c_int(kind), ctypes.c_int(d), _ptr(u), _ptr(v), _ptr(f))
+    return f
+
+
+def logmean(a: float, b: float) -> float:
+    return lib().orc_logmean(a, b)
+
+
+def cons2evec(cfg: Cfg, u) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    w = np.zeros_like(u)
+    lib().orc_cons2evec(ctypes.byref(cfg), _ptr(u), _ptr(w))
+    return w
+
+
+def entropy(cfg: Cfg, u) -> float:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return lib().orc_entropy(ctypes.byref(cfg), _ptr(u))
+
+
+def isvalid(cfg: Cfg, u) -> bool:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return bool(lib().orc_isvalid(ctypes.byref(cfg), _ptr(u)))
+
+
+# ---- element operators ---------------------------------------------------
+
+def nvars(cfg: Cfg) -> int:
+    return 9 if cfg.eq == EQ_MHD else 5
+
+
+def indicator(cfg: Cfg, u: np.ndarray):
+    nelem = u.shape[0]
+    a = np.zeros(nelem)
+    e = np.zeros(nelem)
+    lib().orc_indicator(ctypes.byref(cfg), nelem, _ptr(np.ascontiguousarray(u)), _ptr(a), _ptr(e))
+    return a, e
+
+
+def rhs(cfg: Cfg, nbr: np.ndarray, delta: np.ndarray, u: np.ndarray,
+        alpha: Optional[np.ndarray] = None) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    ap = _ptr(np.ascontiguousarray(alpha, dtype=np.float64)) if alpha is not None else None
+    lib().orc_rhs(ctypes.byref(cfg), u.shape[0], _iptr(np.ascontiguousarray(nbr, dtype=np.int32)),
+                  _ptr(np.ascontiguousarray(delta, dtype=np.float64)), _ptr(u), ap, _ptr(out))
+    return out
+
+
+def calc_timestep(cfg: Cfg, delta: np.ndarray, u: np.ndarray):
+    lm = np.zeros(1)
+    vm = np.zeros(1)
+    dt = lib().orc_calc_timestep(ctypes.byref(cfg), u.shape[0],
+                                 _ptr(np.ascontiguousarray(delta, dtype=np.float64)),
+                                 _ptr(np.ascontiguousarray(u)), _ptr(lm), _ptr(vm))
+    return dt, float(lm[0]), float(vm[0])
+
+
+def repair(cfg: Cfg, ue: np.ndarray) -> bool:
+    ue = np.ascontiguousarray(ue, dtype=np.float64)
+    ok = lib().orc_repair(ctypes.byref(cfg), _ptr(ue))
+    return bool(ok), ue
+
+
+def step(cfg: Cfg, nbr, delta, u: np.ndarray, r0: np.ndarray, r1: np.ndarray, dt: float) -> int:
+    """One time step in place on u (registers r0/r1 updated in place)."""
+    for a in (u, r0, r1):
+        assert a.flags.c_contiguous and a.dtype == np.float64
+    return lib().orc_step(ctypes.byref(cfg), u.shape[0], _iptr(np.ascontiguousarray(nbr, dtype=np.int32)),
+                          _ptr(np.ascontiguousarray(delta, dtype=np.float64)), _ptr(u), _ptr(r0), _ptr(r1),
+                          ctypes.c_double(dt))
+
+
+def stage(cfg: Cfg, nbr, delta, u: np.ndarray, r0: np.ndarray, r1: np.ndarray, s: int, dt: float) -> int:
+    """One RK stage in place on u (registers r0/r1 updated in place)."""
+    for a in (u, r0, r1):
+        assert a.flags.c_contiguous and a.dtype == np.float64
+    return lib().orc_stage(ctypes.byref(cfg), u.shape[0], _iptr(np.ascontiguousarray(nbr, dtype=np.int32)),
+                           _ptr(np.ascontiguousarray(delta, dtype=np.float64)), _ptr(u), _ptr(r0), _ptr(r1),
+                           int(s), ctypes.c_double(dt))
+
+
+def run(cfg: Cfg, nbr, delta, u: np.ndarray, nsteps: int):
+    """nsteps CFL-limited steps in place on u; returns (rc, dt history)."""
+    assert u.flags.c_contiguous and u.dtype == np.float64
+    hist = np.zeros(nsteps)
+    rc = lib().orc_run(ctypes.byref(cfg), u.shape[0], _iptr(np.ascontiguousarray(nbr, dtype=np.int32)),
+                       _ptr(np.ascontiguousarray(delta, dtype=np.float64)), _ptr(u), nsteps, _ptr(hist))
+    return rc, hist
+
+
+def lsrk54_coeffs():
+    a, b, c = np.zeros(5), np.zeros(5), np.zeros(5)
+    lib().orc_lsrk54_coeffs(_ptr(a), _ptr(b), _ptr(c))
+    return a, b, c
+
+
+def ssprk54_coeffs():
+    k = np.zeros(23)
+    lib().orc_ssprk54_coeffs(_ptr(k))
+    return k
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# ---- the reference's own Fortran leaf modules (oracle/_ref) ---------------
+
+class Ref:
+    """The reference's Fortran modules compiled in place (oracle/_ref)."""
+
+    def __init__(self):
+        if not (os.path.exists(_REF_A) and os.path.exists(_REF_B)):
+            raise FileNotFoundError("oracle/_ref not built (make -C oracle ref)")
+        self.a = ctypes.CDLL(_REF_A)
+        self.b = ctypes.CDLL(_REF_B)
+        self.a.ref_pressure.restype = ctypes.c_double
+        self.a.ref_lambda_max.restype = ctypes.c_double
+        self.a.ref_lambda_max.argtypes = [ctypes.c_int, _P]
+        self.a.ref_flux.argtypes = [ctypes.c_int, _P, _P]
+        self.a.ref_hllc.argtypes = [ctypes.c_int, _P, _P, _P]
+        self.a.ref_ec.argtypes = [ctypes.c_int, _P, _P, _P]
+        self.a.ref_lgl.argtypes = [ctypes.c_int, _P, _P]
+        self.a.ref_diff_matrix.argtypes = [ctypes.c_int, _P, _P]
+        self.a.ref_legendre.argtypes = [ctypes.c_int, ctypes.c_double, _P, _P]
+        self.a.ref_vandermonde.argtypes = [ctypes.c_int, _P, _P, _P]
+        self.a.ref_minmod3.restype = ctypes.c_double
+        self.a.ref_minmod3.argtypes = [ctypes.c_double] * 3
+        self.a.ref_isvalid.restype = ctypes.c_int
+        self.b.ref_rusanov.argtypes = [ctypes.c_int, _P, _P, _P]
+        self.b.ref_central.argtypes = [ctypes.c_int, _P, _P, _P]
+
+    @staticmethod
+    def _v(u):
+        return np.ascontiguousarray(u, dtype=np.float64)
+
+    def pressure(self, u):
+        return self.a.ref_pressure(_ptr(self._v(u)))
+
+    def flux(self, d, u):
+        f = np.zeros(5)
+        self.a.ref_flux(d, _ptr(self._v(u)), _ptr(f))
+        return f
+
+    def lambda_max(self, d, u):
+        return self.a.ref_lambda_max(d, _ptr(self._v(u)))
+
+    def cons2evec(self, u):
+        w = np.zeros(5)
+        self.a.ref_cons2evec(_ptr(self._v(u)), _ptr(w))
+        return w
+
+    def isvalid(self, u):
+        return bool(self.a.ref_isvalid(_ptr(self._v(u))))
+
+    def hllc(self, d, uL, uR):
+        f = np.zeros(5)
+        self.a.ref_hllc(d, _ptr(self._v(uL)), _ptr(self._v(uR)), _ptr(f))
+        return f
+
+    def rusanov(self, d, uL, uR):
+        f = np.zeros(5)
+        self.b.ref_rusanov(d, _ptr(self._v(uL)), _ptr(self._v(uR)), _ptr(f))
+        return f
+
+    def ec(self, d, u, v):
+        f = np.zeros(5)
+        self.a.ref_ec(d, _ptr(self._v(u)), _ptr(self._v(v)), _ptr(f))
+        return f
+
+    def central(self, d, u, v):
+        f = np.zeros(5)
+        self.b.ref_central(d, _ptr(self._v(u)), _ptr(self._v(v)), _ptr(f))
+        return f
+
+    def lsrk_coeffs(self):
+        a, b, c = np.zeros(5), np.zeros(5), np.zeros(5)
+        self.a.ref_lsrk_coeffs(_ptr(a), _ptr(b), _ptr(c))
+        return a, b, c
+
+    def ssp_coeffs(self):
+        k = np.zeros(23)
+        self.b.ref_ssp_coeffs(_ptr(k))
+        return k
+
+    def lgl(self, N):
+        x, w = np.zeros(N + 1), np.zeros(N + 1)
+        self.a.ref_lgl(N, _ptr(x), _ptr(w))
+        return x, w
+
+    def diff_matrix(self, N, x):
+        D = np.zeros((N + 1, N + 1), order="F")
+        self.a.ref_diff_matrix(N, _ptr(self._v(x)), D.ctypes.data_as(_P))
+        return np.ascontiguousarray(D)
+
+    def legendre(self, n, x):
+        L, dL = np.zeros(1), np.zeros(1)
+        self.a.ref_legendre(n, x, _ptr(L), _ptr(dL))
+        return float(L[0]), float(dL[0])
+
+    def vandermonde(self, x):
+        m = len(x)
+        V = np.zeros((m, m), order="F")
+        Vi = np.zeros((m, m), order="F")
+        self.a.ref_vandermonde(m, _ptr(self._v(x)), V.ctypes.data_as(_P), Vi.ctypes.data_as(_P))
+        return np.ascontiguousarray(V), np.ascontiguousarray(Vi)
+
+    def minmod3(self, a, b, c):
+        return self.a.ref_minmod3(a, b, c)
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF_A) and os.path.exists(_REF_B)

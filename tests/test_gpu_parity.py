@@ -1,0 +1,166 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 CPU oracle.
+
+Bars (from the task statement of the plain FP64 path):
+  * RHS per call:        max|rhs_gpu - rhs_orc| / max|rhs_orc| <= 1e-12
+  * after 100 RK steps:  max|u_gpu - u_orc|     / max|u_orc|   <= 1e-10
+Both runs start from the same host-generated initial data on the same mesh.
+"""
+import numpy as np
+import pytest
+
+from nemo_amd import _lib, mesh as meshmod, setups, solver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+RUN_TOL = 1e-10
+
+
+def _cfgs(N, vol, surf, ind, td, repair=0, cfl=0.5):
+    g = solver.make_config(N=N, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair, cfl=cfl)
+    o = O.make_cfg(N=N, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair, cfl=cfl)
+    return g, o
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+COMBOS = [
+    # (N, volume, surface, indicator, timedisc, setup)
+    (3, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),   # config 1
+    (3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),               # config 2
+    (3, _lib.VOL_CENTRAL, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_LSRK54, "vortex"),
+    (3, _lib.VOL_EC, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "kh"),
+    (3, _lib.VOL_WEAK, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),
+    (2, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),
+    (4, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),
+    (2, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),
+    (4, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),
+]
+
+
+def _setup(name, n, N):
+    m = meshmod.uniform_periodic(n)
+    if name == "vortex":
+        return m, setups.isentropic_vortex(m, N)
+    return m, setups.kelvin_helmholtz(m, N, seed=1234)
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=lambda c: f"N{c[0]}-vol{c[1]}-surf{c[2]}-ind{c[3]}-td{c[4]}-{c[5]}")
+def test_rhs_parity(combo):
+    N, vol, surf, ind, td, name = combo
+    m, u0 = _setup(name, 8, N)
+    g, o = _cfgs(N, vol, surf, ind, td)
+    print(f"rhs parity {combo}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        r = s.kernel_calc_flux()
+        a = s.indicator()
+    aref, _ = O.indicator(o, u0) if ind == _lib.IND_PP else (np.zeros(m.nelem), None)
+    rref = O.rhs(o, m.nbr, m.delta, u0, aref if ind == _lib.IND_PP else None)
+    assert np.array_equal(a > 0, aref > 0)
+    assert np.abs(a - aref).max() <= 1e-12
+    if ind == _lib.IND_PP:
+        assert (aref > 0).any(), "blending never active: the FV path is not exercised"
+    err = _rel(r, rref)
+    print(f"  rel Linf {err:.3e}")
+    assert err <= RHS_TOL
+
+
+def test_timestep_parity():
+    m, u0 = _setup("kh", 8, 3)
+    g, o = _cfgs(3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        dt, lm, vm = s.hook_calc_timestep()
+    dto, lmo, vmo = O.calc_timestep(o, m.delta, u0)
+    assert abs(dt - dto) <= 1e-15 * dto
+    assert abs(lm - lmo) <= 1e-15 * lmo
+    assert abs(vm - vmo) <= 1e-15 * vmo
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_run_100_steps(which):
+    if which == "c1":
+        N, vol, surf, ind, td, name = COMBOS[0]
+    else:
+        N, vol, surf, ind, td, name = COMBOS[1]
+    m, u0 = _setup(name, 8, N)
+    g, o = _cfgs(N, vol, surf, ind, td, repair=1)
+    print(f"100-step parity {which}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        hist = s.run(100, record_dt=True)
+        u = s.get_state()
+    uref = u0.copy()
+    rc, href = O.run(o, m.nbr, m.delta, uref, 100)
+    assert rc == 0
+    assert np.abs(hist - href).max() <= 1e-12 * href.max()
+    err = _rel(u, uref)
+    print(f"  rel Linf after 100 steps {err:.3e}")
+    assert err <= RUN_TOL
+
+
+def test_stage_api_matches_run():
+    """nemo_gpu_step with a host dt equals nemo_gpu_run's device-side dt path."""
+    N, vol, surf, ind, td, name = COMBOS[1]
+    m, u0 = _setup(name, 4, N)
+    g, _ = _cfgs(N, vol, surf, ind, td, repair=1)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        h = s.run(3, record_dt=True)
+        ua = s.get_state()
+        s.set_state(u0)
+        for dt in h:
+            s.step(float(dt))
+        s.synchronize()
+        ub = s.get_state()
+    assert np.array_equal(ua, ub)
+
+
+def _rough_state(m, N, seed=3):
+    """Random primitive state with pressure near zero (repair territory)."""
+    rng = np.random.default_rng(seed)
+    M3 = (N + 1) ** 3
+    rho = rng.uniform(0.5, 2.0, (m.nelem, M3))
+    v = rng.uniform(-1.0, 1.0, (3, m.nelem, M3))
+    p = rng.uniform(1e-3, 2e-3, (m.nelem, M3))
+    return np.ascontiguousarray(setups.prim2cons(rho, v[0], v[1], v[2], p, 1.4))
+
+
+def _pressure(u, gamma=1.4):
+    return (gamma - 1.0) * (u[:, 4] - 0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2 + u[:, 3] ** 2) / u[:, 0])
+
+
+def test_repair_parity():
+    """Stage 1 with 4x the CFL step leaves nodes with p <= 0; the device
+    repair (tile squeeze) must reproduce the oracle's repaired state."""
+    N = 3
+    m = meshmod.uniform_periodic(4)
+    u0 = _rough_state(m, N)
+    g, o = _cfgs(N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, repair=1)
+    dt = 4.0 * O.calc_timestep(o, m.delta, u0)[0]
+    # the unrepaired stage really is invalid somewhere
+    o_norep = O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP, timedisc=O.TD_LSRK54)
+    ua = u0.copy()
+    O.stage(o_norep, m.nbr, m.delta, ua, np.zeros_like(ua), np.zeros_like(ua), 1, dt)
+    assert ((ua[:, 0] <= 0) | (_pressure(ua) <= 0)).sum() > 0
+    uref = u0.copy()
+    assert O.stage(o, m.nbr, m.delta, uref, np.zeros_like(uref), np.zeros_like(uref), 1, dt) == 0
+    print("repair parity", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        s._call("nemo_gpu_stage", s._h, 1, dt)
+        s.synchronize()
+        u = s.get_state()
+    assert ((u[:, 0] > 0) & (_pressure(u) > 0)).all()
+    err = _rel(u, uref)
+    print(f"  rel Linf {err:.3e}")
+    assert err <= 1e-10

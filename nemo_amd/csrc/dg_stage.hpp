@@ -65,6 +65,7 @@ struct StageArgs {
     const double* __restrict__ delta;// [nelem][3]
     const double* __restrict__ dt_dev;
     int* __restrict__ status;
+    double* __restrict__ cfl_part;   // non-null: per-block CFL partials of the new state
     long long nelem;
     double dt;
     int stage;      // 1..5
@@ -83,7 +84,8 @@ __device__ __forceinline__ constexpr int nidx(int l, int a, int b) {
 }
 
 template <int SURF, int D, typename Real>
-__device__ __forceinline__ void riemann(const Gas<Real>& g, const Real* uL, const Real* uR, Real* f) {
+__device__ __forceinline__ void riemann(const Gas<Real>& g, const NodeState<Real>& uL, const NodeState<Real>& uR,
+                                        Real* f) {
     if constexpr (SURF == SURF_HLLC)
         riemann_hllc<D>(g, uL, uR, f);
     else
@@ -108,12 +110,15 @@ struct StageShape {
 };
 
 // One direction pass: contributions of direction D to the node accumulators.
+// The thread's M line nodes (and the two neighbour face nodes) become node
+// states once; the surface Riemann problems, the M(M-1)/2 two-point fluxes
+// and the M-1 FV interface fluxes all read those states.
 template <int D, int M, int BE, int VOL, int SURF, typename Real>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, Real (*s_u)[BE * M * M * M],
-                                               Real (*s_prim)[BE * M * M * M], Real (*s_acc)[BE * M * M * M],
-                                               const Real* s_alpha) {
+                                               Real (*s_acc)[BE * M * M * M], const Real* s_alpha) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
+    constexpr bool BETA = (VOL == VOL_EC);
     const int tid = threadIdx.x;
     const int el = tid / M2;
     const int line = tid - el * M2;
@@ -124,36 +129,42 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     const Gas<Real>& g = a.gas;
     const int base = el * M3;
 
-    Real uf0[5], ufN[5], unm[5], unp[5];
+    NodeState<Real> S[M];
 #pragma unroll
-    for (int h = 0; h < 5; ++h) {
-        uf0[h] = s_u[h][base + nidx<D, M>(0, ta, tb)];
-        ufN[h] = s_u[h][base + nidx<D, M>(M - 1, ta, tb)];
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+        for (int h = 0; h < 5; ++h) S[i].u[h] = s_u[h][base + nidx<D, M>(i, ta, tb)];
+        make_state<D, BETA>(g, S[i]);
     }
     Real jac = Real(0), sdx = Real(0);
-    if (valid) {
-        const long long em = a.nbr[e * 6 + 2 * D];
-        const long long ep = a.nbr[e * 6 + 2 * D + 1];
-        const double dx = a.delta[e * 3 + D];
-        jac = Real(2.0 / dx);
-        sdx = Real(double(M) / dx);
-        const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
-        const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
-#pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            unm[h] = Real(um[h * EM3]);
-            unp[h] = Real(up[h * EM3]);
-        }
-    } else {
-#pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            unm[h] = uf0[h];
-            unp[h] = ufN[h];
-        }
-    }
     Real fm[5], fp[5];
-    riemann<SURF, D>(g, unm, uf0, fm);
-    riemann<SURF, D>(g, ufN, unp, fp);
+    {
+        NodeState<Real> Nm, Np;
+        if (valid) {
+            const long long em = a.nbr[e * 6 + 2 * D];
+            const long long ep = a.nbr[e * 6 + 2 * D + 1];
+            const double dx = a.delta[e * 3 + D];
+            jac = Real(2.0 / dx);
+            sdx = Real(double(M) / dx);
+            const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
+            const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
+#pragma unroll
+            for (int h = 0; h < 5; ++h) {
+                Nm.u[h] = Real(um[h * EM3]);
+                Np.u[h] = Real(up[h * EM3]);
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 5; ++h) {
+                Nm.u[h] = S[0].u[h];
+                Np.u[h] = S[M - 1].u[h];
+            }
+        }
+        make_state<D, false>(g, Nm);
+        make_state<D, false>(g, Np);
+        riemann<SURF, D>(g, Nm, S[0], fm);
+        riemann<SURF, D>(g, S[M - 1], Np, fp);
+    }
 
     Real c[M][5];
     if constexpr (VOL == VOL_WEAK) {
@@ -162,10 +173,7 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         Real fx[M][5];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            Real uj[5];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) uj[h] = s_u[h][base + nidx<D, M>(j, ta, tb)];
-            flux<D>(g, uj, fx[j]);
+            flux_s<D>(S[j], fx[j]);
 #pragma unroll
             for (int h = 0; h < 5; ++h) fx[j][h] = sdx * fx[j][h];
         }
@@ -183,54 +191,27 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
             }
     } else {
         // Split-form flux differencing (Gassner/Winters/Kopriva 2016):
-        // rhs_i = -jac [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ]
+        // rhs_i = -jac [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ];
+        // each pair is evaluated once and used by both nodes (f# symmetric).
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
             for (int h = 0; h < 5; ++h) c[i][h] = Real(0);
-        if constexpr (VOL == VOL_EC) {
-            NodePrim<Real> P[M];
 #pragma unroll
-            for (int i = 0; i < M; ++i) {
-                const int n = base + nidx<D, M>(i, ta, tb);
-                P[i].rho = s_u[DENS][n];
-                P[i].v[0] = s_prim[0][n];
-                P[i].v[1] = s_prim[1][n];
-                P[i].v[2] = s_prim[2][n];
-                P[i].p = s_prim[3][n];
-                P[i].beta = s_prim[4][n];
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int m = i + 1; m < M; ++m) {
+                Real F[5];
+                if constexpr (VOL == VOL_EC)
+                    ec_flux<D>(g, S[i], S[m], F);
+                else
+                    central_flux<D>(S[i], S[m], F);
+#pragma unroll
+                for (int h = 0; h < 5; ++h) {
+                    c[i][h] += a.op.S[i][m] * F[h];
+                    c[m][h] += a.op.S[m][i] * F[h];
+                }
             }
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-#pragma unroll
-                for (int m = i + 1; m < M; ++m) {
-                    Real F[5];
-                    ec_flux<D>(g, P[i], P[m], F);
-#pragma unroll
-                    for (int h = 0; h < 5; ++h) {
-                        c[i][h] += a.op.S[i][m] * F[h];
-                        c[m][h] += a.op.S[m][i] * F[h];
-                    }
-                }
-        } else {
-            Real ul[M][5];
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-#pragma unroll
-                for (int h = 0; h < 5; ++h) ul[i][h] = s_u[h][base + nidx<D, M>(i, ta, tb)];
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-#pragma unroll
-                for (int m = i + 1; m < M; ++m) {
-                    Real F[5];
-                    central_flux<D>(g, ul[i], ul[m], F);
-#pragma unroll
-                    for (int h = 0; h < 5; ++h) {
-                        c[i][h] += a.op.S[i][m] * F[h];
-                        c[m][h] += a.op.S[m][i] * F[h];
-                    }
-                }
-        }
 #pragma unroll
         for (int h = 0; h < 5; ++h) {
             c[0][h] = c[0][h] - fm[h] * a.op.inv_w[0];
@@ -248,12 +229,9 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     const Real alpha = s_alpha[el];
     if (alpha > Real(0)) {
         const Real beta = Real(1) - alpha;
-        Real Fprev[5], ui[5], un[5];
+        Real Fprev[5];
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            Fprev[h] = fm[h];
-            ui[h] = uf0[h];
-        }
+        for (int h = 0; h < 5; ++h) Fprev[h] = fm[h];
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             Real Fnext[5];
@@ -261,16 +239,13 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
 #pragma unroll
                 for (int h = 0; h < 5; ++h) Fnext[h] = fp[h];
             } else {
-#pragma unroll
-                for (int h = 0; h < 5; ++h) un[h] = s_u[h][base + nidx<D, M>(i + 1, ta, tb)];
-                riemann<SURF, D>(g, ui, un, Fnext);
+                riemann<SURF, D>(g, S[i], S[i + 1], Fnext);
             }
-            const Real s = -jac * a.op.inv_w[i];
+            const Real sc = -jac * a.op.inv_w[i];
 #pragma unroll
             for (int h = 0; h < 5; ++h) {
-                c[i][h] = beta * c[i][h] + alpha * (s * (Fnext[h] - Fprev[h]));
+                c[i][h] = beta * c[i][h] + alpha * (sc * (Fnext[h] - Fprev[h]));
                 Fprev[h] = Fnext[h];
-                if (i < M - 1) ui[h] = un[h];
             }
         }
     }
@@ -294,15 +269,19 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
 // 0.5*10^(-1.8*M^0.25), clipped by alpha_min/alpha_max.
 template <int M, int BE, typename Real>
 __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long long e0, Real (*s_u)[BE * M * M * M],
-                                             Real (*s_acc)[BE * M * M * M], Real (*s_part)[3], Real* s_alpha,
-                                             const Real* s_p) {
+                                             Real (*s_acc)[BE * M * M * M], Real* s_alpha) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
     const int tid = threadIdx.x;
     Real* q = s_acc[2];
     Real* t1 = s_acc[0];
     Real* t2 = s_acc[1];
-    for (int idx = tid; idx < NN; idx += NT) q[idx] = s_u[DENS][idx] * s_p[idx];
+    for (int idx = tid; idx < NN; idx += NT) {
+        Real uu[5];
+#pragma unroll
+        for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
+        q[idx] = uu[DENS] * pressure(a.gas, uu);
+    }
     __syncthreads();
     const int el = tid / M2, line = tid - el * M2;
     const int ta = line % M, tb = line / M;
@@ -348,17 +327,18 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
             if (ta < M - 1 && tb < M - 1 && r < M - 1) c1 += e2;
             if (ta < M - 2 && tb < M - 2 && r < M - 2) c2 += e2;
         }
-        s_part[tid][0] = tot;
-        s_part[tid][1] = c1;
-        s_part[tid][2] = c2;
+        // q (row 2) is dead after the x contraction; rows 2..4 take the partials
+        s_acc[2][tid] = tot;
+        s_acc[3][tid] = c1;
+        s_acc[4][tid] = c2;
     }
     __syncthreads();
     if (line == 0) {
         Real tot = Real(0), c1 = Real(0), c2 = Real(0);
         for (int l = 0; l < M2; ++l) {
-            tot += s_part[el * M2 + l][0];
-            c1 += s_part[el * M2 + l][1];
-            c2 += s_part[el * M2 + l][2];
+            tot += s_acc[2][el * M2 + l];
+            c1 += s_acc[3][el * M2 + l];
+            c2 += s_acc[4][el * M2 + l];
         }
         double energy = 0.0;
         if (tot > Real(0)) {
@@ -486,22 +466,94 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
     return all_ok();
 }
 
+// CFL quantities of one block of elements held in LDS (kernel/dgsem/weakform/
+// 2d/kernel_timestep_mod.f90:65-121 with the 3-D minimum of kernel/mcfv/3d/
+// kernel_timestep_mod.f90:60-106): per element dt_e = min_d delta_d /
+// max_nodes lambda_d, the fluid speed maximum for GLM cleaning, reduced to
+// one partial (dt_min, lambda_max, v_max) per block.  Thread t first folds
+// the M nodes of x-line t, then one thread per element folds its lines.
+template <int M, int BE, int R>
+__device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __restrict__ delta, long long e0,
+                                          long long nelem, double (*s_u)[BE * M * M * M], double (*s_w)[R],
+                                          double* __restrict__ out) {
+    static_assert(R >= BE * M * M, "work rows must hold one value per thread");
+    constexpr int M2 = M * M, M3 = M2 * M;
+    const int tid = threadIdx.x;
+    const int el = tid / M2, line = tid - el * M2;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, vm = 0.0;
+    if (e0 + el < nelem) {
+        for (int i = 0; i < M; ++i) {
+            const int n = el * M3 + i + M * line;
+            double uu[5];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][n];
+            const double a0 = lambda_max<0>(g, uu), a1 = lambda_max<1>(g, uu), a2 = lambda_max<2>(g, uu);
+            l0 = a0 > l0 ? a0 : l0;
+            l1 = a1 > l1 ? a1 : l1;
+            l2 = a2 > l2 ? a2 : l2;
+            const double v0 = fabs(uu[1] / uu[0]), v1 = fabs(uu[2] / uu[0]), v2 = fabs(uu[3] / uu[0]);
+            double v = v0 > v1 ? v0 : v1;
+            v = v > v2 ? v : v2;
+            vm = v > vm ? v : vm;
+        }
+    }
+    __syncthreads();  // the work rows may still be read before this point
+    s_w[0][tid] = l0;
+    s_w[1][tid] = l1;
+    s_w[2][tid] = l2;
+    s_w[3][tid] = vm;
+    __syncthreads();
+    double dte = 1.7976931348623157e308, lme = 0.0, vme = 0.0;
+    if (tid < BE && e0 + tid < nelem) {
+        const long long e = e0 + tid;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        for (int l = 0; l < M2; ++l) {
+            const int r = tid * M2 + l;
+            m0 = s_w[0][r] > m0 ? s_w[0][r] : m0;
+            m1 = s_w[1][r] > m1 ? s_w[1][r] : m1;
+            m2 = s_w[2][r] > m2 ? s_w[2][r] : m2;
+            vme = s_w[3][r] > vme ? s_w[3][r] : vme;
+        }
+        const double t0 = delta[e * 3 + 0] / m0, t1 = delta[e * 3 + 1] / m1, t2 = delta[e * 3 + 2] / m2;
+        dte = t0 < t1 ? t0 : t1;
+        dte = dte < t2 ? dte : t2;
+        lme = m0 > m1 ? m0 : m1;
+        lme = lme > m2 ? lme : m2;
+    }
+    __syncthreads();
+    if (tid < BE) {
+        s_w[0][tid] = dte;
+        s_w[1][tid] = lme;
+        s_w[2][tid] = vme;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double dtb = 1.7976931348623157e308, lmb = 0.0, vmb = 0.0;
+        for (int k = 0; k < BE; ++k) {
+            dtb = s_w[0][k] < dtb ? s_w[0][k] : dtb;
+            lmb = s_w[1][k] > lmb ? s_w[1][k] : lmb;
+            vmb = s_w[2][k] > vmb ? s_w[2][k] : vmb;
+        }
+        out[0] = dtb;
+        out[1] = lmb;
+        out[2] = vmb;
+    }
+}
+
 template <int M, int BE, int VOL, int SURF>
 __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> a) {
     using Real = double;
     using Sh = StageShape<M, BE>;
     constexpr int M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
-    constexpr int NPRIM = (VOL == VOL_EC) ? 5 : 1;
     __shared__ Real s_u[5][NN];
-    __shared__ Real s_prim[NPRIM][NN];
     __shared__ Real s_acc[5][NN];
-    __shared__ Real s_part[NT][3];
     __shared__ Real s_alpha[BE];
     __shared__ int s_bad[BE];
 
     const int tid = threadIdx.x;
     const long long nblocks = (a.nelem + BE - 1) / BE;
-    const long long e0 = xcd_block(nblocks) * BE;
+    const long long blk = xcd_block(nblocks);
+    const long long e0 = blk * BE;
     const long long EM3 = a.nelem * (long long)M3;
     const long long nvalid_nodes = (a.nelem - e0 < BE ? a.nelem - e0 : BE) * (long long)M3;
 
@@ -516,33 +568,14 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
     }
     __syncthreads();
 
-    // ---- per-node primitives (two-point flux and indicator) -------------
-    if (VOL == VOL_EC || a.indicator == IND_PP) {
-        for (int idx = tid; idx < NN; idx += NT) {
-            Real uu[5];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
-            const NodePrim<Real> q = node_prim(a.gas, uu);
-            if constexpr (VOL == VOL_EC) {
-                s_prim[0][idx] = q.v[0];
-                s_prim[1][idx] = q.v[1];
-                s_prim[2][idx] = q.v[2];
-                s_prim[3][idx] = q.p;
-                s_prim[4][idx] = q.beta;
-            } else {
-                s_prim[0][idx] = q.p;
-            }
-        }
-        __syncthreads();
-    }
-    if (a.indicator == IND_PP) indicator_pp<M, BE, Real>(a, e0, s_u, s_acc, s_part, s_alpha, s_prim[NPRIM == 5 ? 3 : 0]);
+    if (a.indicator == IND_PP) indicator_pp<M, BE, Real>(a, e0, s_u, s_acc, s_alpha);
 
     // ---- three direction passes ----------------------------------------
-    direction_pass<0, M, BE, VOL, SURF, Real>(a, e0, s_u, s_prim, s_acc, s_alpha);
+    direction_pass<0, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
     __syncthreads();
-    direction_pass<1, M, BE, VOL, SURF, Real>(a, e0, s_u, s_prim, s_acc, s_alpha);
+    direction_pass<1, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
     __syncthreads();
-    direction_pass<2, M, BE, VOL, SURF, Real>(a, e0, s_u, s_prim, s_acc, s_alpha);
+    direction_pass<2, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
     __syncthreads();
 
     // ---- RHS only (parity path) -----------------------------------------
@@ -613,70 +646,65 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
                 else if (s == 3) a.r1[gi] = a.r1[gi] + a.rk.ssp[21] * st;
             }
         }
+
+    // ---- CFL partials of the new state (last stage of a step) -----------
+    if (a.cfl_part) block_cfl<M, BE, NN>(a.gas, a.delta, e0, a.nelem, s_u, s_acc, a.cfl_part + 4 * blk);
 }
 
-// CFL reduction: kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-121 with
-// the 3-D per-direction minimum of kernel/mcfv/3d/kernel_timestep_mod.f90:
-// 60-106.  Per element dt_e = min_d delta_d / max_nodes lambda_d; the global
-// minimum is taken with an integer atomicMin on the IEEE bits (positive
-// doubles order like their bit patterns), so the result does not depend on
-// scheduling.  lmax and vmax (GLM cleaning speed) likewise with atomicMax.
+// Standalone CFL reduction of a state in HBM (first step, calc_timestep):
+// the same per-block partials as the fused stage epilogue.
 template <int M, int BE>
-__global__ __launch_bounds__(BE * M * M * M) void cfl_kernel(const double* __restrict__ u, const double* __restrict__ delta,
-                                                             long long nelem, Gas<double> g,
-                                                             unsigned long long* __restrict__ red) {
-    constexpr int M3 = M * M * M, NN = BE * M3;
-    __shared__ double s_l[4][NN];
+__global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restrict__ u, const double* __restrict__ delta,
+                                                         long long nelem, Gas<double> g, double* __restrict__ part) {
+    constexpr int M3 = M * M * M, NN = BE * M3, NT = BE * M * M;
+    __shared__ double s_u[5][NN];
+    __shared__ double s_w[4][NT];
     const int tid = threadIdx.x;
-    const long long e0 = (long long)blockIdx.x * BE;
+    const long long blk = blockIdx.x;
+    const long long e0 = blk * BE;
     const long long EM3 = nelem * (long long)M3;
-    const long long e = e0 + tid / M3;
-    if (e < nelem) {
-        double uu[5];
+    const long long nvalid = (nelem - e0 < BE ? nelem - e0 : BE) * (long long)M3;
 #pragma unroll
-        for (int h = 0; h < 5; ++h) uu[h] = u[h * EM3 + e0 * M3 + tid];
-        s_l[0][tid] = lambda_max<0>(g, uu);
-        s_l[1][tid] = lambda_max<1>(g, uu);
-        s_l[2][tid] = lambda_max<2>(g, uu);
-        const double v0 = fabs(uu[1] / uu[0]), v1 = fabs(uu[2] / uu[0]), v2 = fabs(uu[3] / uu[0]);
-        const double vm = v0 > v1 ? v0 : v1;
-        s_l[3][tid] = vm > v2 ? vm : v2;
-    }
+    for (int h = 0; h < 5; ++h)
+        for (int idx = tid; idx < NN; idx += NT)
+            s_u[h][idx] = idx < nvalid ? u[h * EM3 + e0 * M3 + idx] : (h == 0 || h == 4 ? 1.0 : 0.0);
     __syncthreads();
-    if (tid < BE && e0 + tid < nelem) {
-        const int b = tid * M3;
-        const long long ee = e0 + tid;
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0, vm = 0.0;
-        for (int n = 0; n < M3; ++n) {
-            l0 = s_l[0][b + n] > l0 ? s_l[0][b + n] : l0;
-            l1 = s_l[1][b + n] > l1 ? s_l[1][b + n] : l1;
-            l2 = s_l[2][b + n] > l2 ? s_l[2][b + n] : l2;
-            vm = s_l[3][b + n] > vm ? s_l[3][b + n] : vm;
-        }
-        const double t0 = delta[ee * 3 + 0] / l0, t1 = delta[ee * 3 + 1] / l1, t2 = delta[ee * 3 + 2] / l2;
-        double tm = t0 < t1 ? t0 : t1;
-        tm = tm < t2 ? tm : t2;
-        double lm = l0 > l1 ? l0 : l1;
-        lm = lm > l2 ? lm : l2;
-        atomicMin(&red[0], (unsigned long long)__double_as_longlong(tm));
-        atomicMax(&red[1], (unsigned long long)__double_as_longlong(lm));
-        atomicMax(&red[2], (unsigned long long)__double_as_longlong(vm));
-    }
+    block_cfl<M, BE, NT>(g, delta, e0, nelem, s_u, s_w, part + 4 * blk);
 }
 
-// dt = 0.5*CFL/(2*N_NODES-1) * dtmin (kernel_timestep_mod.f90:49); also
-// appends to the dt history and re-arms the reduction words.
-__global__ void cfl_finalize(unsigned long long* red, double cfl, int M, double* dt_dev, double* hist, long long slot) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double dtmin = __longlong_as_double((long long)red[0]);
-        const double dt = 0.5 * cfl / double(2 * M - 1) * dtmin;
-        dt_dev[0] = dt;
-        dt_dev[1] = __longlong_as_double((long long)red[1]);
-        dt_dev[2] = __longlong_as_double((long long)red[2]);
-        if (hist && slot >= 0) hist[slot] = dt;
-        red[0] = 0x7FEFFFFFFFFFFFFFull;  // HUGE(1.0_dp)
-        red[1] = 0ull;
-        red[2] = 0ull;
+// Deterministic reduction of the per-block partials (min and max are exact,
+// so the result does not depend on the order) and
+// dt = 0.5*CFL/(2*N_NODES-1) * dtmin (kernel_timestep_mod.f90:49).
+__global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ part, long long npart, double cfl,
+                                                      int M, double* __restrict__ dt_dev, double* __restrict__ hist,
+                                                      long long slot) {
+    __shared__ double s[3][1024];
+    const int tid = threadIdx.x;
+    double dt = 1.7976931348623157e308, lm = 0.0, vm = 0.0;
+    for (long long i = tid; i < npart; i += 1024) {
+        const double* q = part + 4 * i;
+        dt = q[0] < dt ? q[0] : dt;
+        lm = q[1] > lm ? q[1] : lm;
+        vm = q[2] > vm ? q[2] : vm;
+    }
+    s[0][tid] = dt;
+    s[1][tid] = lm;
+    s[2][tid] = vm;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (tid < off) {
+            s[0][tid] = s[0][tid + off] < s[0][tid] ? s[0][tid + off] : s[0][tid];
+            s[1][tid] = s[1][tid + off] > s[1][tid] ? s[1][tid + off] : s[1][tid];
+            s[2][tid] = s[2][tid + off] > s[2][tid] ? s[2][tid + off] : s[2][tid];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double d = 0.5 * cfl / double(2 * M - 1) * s[0][0];
+        dt_dev[0] = d;
+        dt_dev[1] = s[1][0];
+        dt_dev[2] = s[2][0];
+        if (hist && slot >= 0) hist[slot] = d;
     }
 }
 

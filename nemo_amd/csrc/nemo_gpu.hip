@@ -240,7 +240,7 @@ struct Solver {
     int32_t* nbr = nullptr;
     double* delta = nullptr;
     double* dt_dev = nullptr;  // [3]: dt, lmax, vmax
-    unsigned long long* red = nullptr;  // [3]
+    double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;
     double* hist = nullptr;
     long long hist_cap = 0;
@@ -262,7 +262,8 @@ void free_mesh_buffers(Solver& s) {
     (void)hipFree(s.alpha);
     (void)hipFree(s.nbr);
     (void)hipFree(s.delta);
-    s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = nullptr;
+    (void)hipFree(s.part);
+    s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = s.part = nullptr;
     s.nbr = nullptr;
 }
 
@@ -278,7 +279,7 @@ struct Tile<5> { static constexpr int BE = 2; };
 
 template <int M>
 nemo::StageArgs<M, double> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
-                                     double* alpha_out) {
+                                     double* alpha_out, double* cfl_part) {
     nemo::StageArgs<M, double> a{};
     a.u_in = s.u[s.cur];
     a.u_out = s.u[s.cur ^ 1];
@@ -290,6 +291,7 @@ nemo::StageArgs<M, double> make_args(Solver& s, int stage, double dt, const doub
     a.delta = s.delta;
     a.dt_dev = dt_dev;
     a.status = s.status;
+    a.cfl_part = cfl_part;
     a.nelem = s.nelem;
     a.dt = dt;
     a.stage = stage;
@@ -319,8 +321,9 @@ void launch_stage_t(Solver& s, const nemo::StageArgs<M, double>& a) {
 }
 
 template <int M>
-int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out) {
-    const auto a = make_args<M>(s, stage, dt, dt_dev, rhs_out, alpha_out);
+int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
+                   double* cfl_part) {
+    const auto a = make_args<M>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
 #define NEMO_CASE(V_, S_)                              \
     if (vol == V_ && surf == S_) {                     \
@@ -337,7 +340,8 @@ int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double
     return fail(NEMO_ERR_ARG, "unsupported volume/surface combination");
 }
 
-int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out) {
+int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
+                 double* cfl_part = nullptr) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s.timing) {
         if (s.ev_used == s.ev.size()) {
@@ -353,9 +357,9 @@ int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* 
     }
     int rc;
     switch (s.M) {
-    case 3: rc = launch_stage_m<3>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
-    case 4: rc = launch_stage_m<4>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
-    case 5: rc = launch_stage_m<5>(s, stage, dt, dt_dev, rhs_out, alpha_out); break;
+    case 3: rc = launch_stage_m<3>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
+    case 4: rc = launch_stage_m<4>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
+    case 5: rc = launch_stage_m<5>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
     default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
     }
     if (rc != NEMO_OK) return rc;
@@ -365,19 +369,39 @@ int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* 
 }
 
 template <int M>
+long long stage_blocks(const Solver& s) {
+    return (s.nelem + Tile<M>::BE - 1) / Tile<M>::BE;
+}
+
+long long nblocks_for(const Solver& s) {
+    switch (s.M) {
+    case 3: return stage_blocks<3>(s);
+    case 4: return stage_blocks<4>(s);
+    default: return stage_blocks<5>(s);
+    }
+}
+
+template <int M>
 void launch_cfl_m(Solver& s) {
-    constexpr int BE = (M == 3) ? 8 : (M == 4 ? 4 : 2);
+    constexpr int BE = Tile<M>::BE;
     nemo::Gas<double> g;
     const double k = s.cfg.gamma;
     g.kappa = k;
     g.kappa_m1 = k - 1.0;
     g.hllc_beta = std::sqrt(0.5 * (k - 1) / k);
     g.half_inv_km1 = 0.5 / (k - 1.0);
-    const long long nblocks = (s.nelem + BE - 1) / BE;
-    hipLaunchKernelGGL((nemo::cfl_kernel<M, BE>), dim3(unsigned(nblocks)), dim3(BE * M * M * M), 0, s.stream,
-                       s.u[s.cur], s.delta, s.nelem, g, s.red);
+    hipLaunchKernelGGL((nemo::cfl_kernel<M, BE>), dim3(unsigned(stage_blocks<M>(s))), dim3(BE * M * M), 0,
+                       s.stream, s.u[s.cur], s.delta, s.nelem, g, s.part);
 }
 
+int launch_finalize(Solver& s, long long hist_slot) {
+    hipLaunchKernelGGL(nemo::cfl_finalize, dim3(1), dim3(1024), 0, s.stream, s.part, nblocks_for(s), s.cfg.cfl, s.M,
+                       s.dt_dev, hist_slot >= 0 ? s.hist : nullptr, hist_slot);
+    HIP_TRY(hipGetLastError());
+    return NEMO_OK;
+}
+
+// CFL time step of the current state into dt_dev (and hist[hist_slot]).
 int launch_cfl(Solver& s, long long hist_slot) {
     switch (s.M) {
     case 3: launch_cfl_m<3>(s); break;
@@ -386,10 +410,7 @@ int launch_cfl(Solver& s, long long hist_slot) {
     default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
     }
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(nemo::cfl_finalize, dim3(1), dim3(64), 0, s.stream, s.red, s.cfg.cfl, s.M, s.dt_dev,
-                       hist_slot >= 0 ? s.hist : nullptr, hist_slot);
-    HIP_TRY(hipGetLastError());
-    return NEMO_OK;
+    return launch_finalize(s, hist_slot);
 }
 
 int check_ready(Solver* s) {
@@ -476,14 +497,11 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         return fail(NEMO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     if (hipMalloc(&s->dt_dev, 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&s->red, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
         (void)hipStreamDestroy(s->stream);
         delete s;
         return fail(NEMO_ERR_HIP, "hipMalloc failed");
     }
-    const unsigned long long init[4] = {0x7FEFFFFFFFFFFFFFull, 0ull, 0ull, 0ull};
-    (void)hipMemcpy(s->red, init, sizeof(init), hipMemcpyHostToDevice);
     (void)hipMemset(s->status, 0, sizeof(int));
     *handle = s;
     return NEMO_OK;
@@ -495,7 +513,6 @@ int nemo_gpu_destroy(void* handle) {
     (void)hipStreamSynchronize(s->stream);
     free_mesh_buffers(*s);
     (void)hipFree(s->dt_dev);
-    (void)hipFree(s->red);
     (void)hipFree(s->status);
     (void)hipFree(s->hist);
     for (auto& p : s->ev) {
@@ -527,6 +544,7 @@ int nemo_gpu_upload_mesh(void* handle, int64_t nelem, const int32_t* nbr, const 
         HIP_TRY(hipMalloc(&s->alpha, size_t(nelem) * sizeof(double)));
         HIP_TRY(hipMalloc(&s->nbr, size_t(nelem) * 6 * sizeof(int32_t)));
         HIP_TRY(hipMalloc(&s->delta, size_t(nelem) * 3 * sizeof(double)));
+        HIP_TRY(hipMalloc(&s->part, size_t(nblocks_for(*s)) * 4 * sizeof(double)));
         HIP_TRY(hipMemsetAsync(s->r0, 0, bytes, s->stream));
         HIP_TRY(hipMemsetAsync(s->r1, 0, bytes, s->stream));
         s->have_state = false;
@@ -628,12 +646,19 @@ int nemo_gpu_run(void* handle, int32_t nsteps, double* dt_hist) {
         HIP_TRY(hipMalloc(&s->hist, size_t(nsteps) * sizeof(double)));
         s->hist_cap = nsteps;
     }
+    // The first step's dt comes from a standalone reduction; every later dt
+    // from the CFL partials that the last stage of the previous step writes.
+    if (nsteps > 0)
+        if (int rc = launch_cfl(*s, dt_hist ? 0 : -1)) return rc;
     for (int it = 0; it < nsteps; ++it) {
-        if (int rc = launch_cfl(*s, dt_hist ? it : -1)) return rc;
+        const bool more = it + 1 < nsteps;
         for (int st = 1; st <= 5; ++st) {
-            if (int rc = launch_stage(*s, st, 0.0, s->dt_dev, nullptr, nullptr)) return rc;
+            double* part = (st == 5 && more) ? s->part : nullptr;
+            if (int rc = launch_stage(*s, st, 0.0, s->dt_dev, nullptr, nullptr, part)) return rc;
             s->cur ^= 1;
         }
+        if (more)
+            if (int rc = launch_finalize(*s, dt_hist ? it + 1 : -1)) return rc;
     }
     if (dt_hist && nsteps > 0)
         HIP_TRY(hipMemcpyAsync(dt_hist, s->hist, size_t(nsteps) * sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -82,35 +82,75 @@ __device__ __forceinline__ Real lambda_max(const Gas<Real>& g, const Real* u) {
     return fabs(u[MN] / u[DENS]) + sqrt(g.kappa * p / u[DENS]);
 }
 
+// One node's state as seen from direction D: everything the two-point flux
+// and the Riemann solvers need, computed once per node and direction pass
+// instead of once per flux evaluation.  The reference recomputes these
+// quantities inside every flux (e.g. two_point_flux_mod.f90:49-61,
+// riemann/hllc/riemann_mod.f90:36-56).  Divisions by rho become products
+// with 1/rho, which moves results by at most an ulp; the pressure keeps the
+// reference's exact expression and rotated summation order (see
+// pressure_ordered) because the HLLC/HLLE blend is not Lipschitz in p_L-p_R.
+template <typename Real>
+struct NodeState {
+    Real u[5];
+    Real inv_rho;
+    Real v[3];   // velocity components x, y, z
+    Real p;      // pressure, rotated order of the pass direction
+    Real beta;   // 0.5*rho/p (two-point flux only)
+};
+
+template <int D, bool BETA, typename Real>
+__device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& s) {
+    s.inv_rho = Real(1) / s.u[DENS];
+    s.v[0] = s.u[MOMX] * s.inv_rho;
+    s.v[1] = s.u[MOMY] * s.inv_rho;
+    s.v[2] = s.u[MOMZ] * s.inv_rho;
+    s.p = pressure_dir<D>(g, s.u);
+    if constexpr (BETA) s.beta = Real(0.5) * s.u[DENS] / s.p;
+}
+
+// xflux (equations_mod.f90:64-79) from a node state.
+template <int D, typename Real>
+__device__ __forceinline__ void flux_s(const NodeState<Real>& s, Real* f) {
+    constexpr int MN = Mom<D, 0>::idx, MT1 = Mom<D, 1>::idx, MT2 = Mom<D, 2>::idx;
+    const Real vn = s.v[D];
+    f[DENS] = s.u[MN];
+    f[MN] = vn * s.u[MN] + s.p;
+    f[MT1] = vn * s.u[MT1];
+    f[MT2] = vn * s.u[MT2];
+    f[ENER] = vn * (s.u[ENER] + s.p);
+}
+
 // Rusanov / local Lax-Friedrichs: equations/euler/riemann/rusanov/riemann_mod.f90:7-24
 template <int D, typename Real>
-__device__ __forceinline__ void riemann_rusanov(const Gas<Real>& g, const Real* uL, const Real* uR, Real* sf) {
+__device__ __forceinline__ void riemann_rusanov(const Gas<Real>& g, const NodeState<Real>& L, const NodeState<Real>& R,
+                                                Real* sf) {
     Real FL[5], FR[5];
-    flux<D>(g, uL, FL);
-    flux<D>(g, uR, FR);
-    const Real a = lambda_max<D>(g, uL), b = lambda_max<D>(g, uR);
+    flux_s<D>(L, FL);
+    flux_s<D>(R, FR);
+    const Real a = fabs(L.v[D]) + sqrt(g.kappa * L.p * L.inv_rho);
+    const Real b = fabs(R.v[D]) + sqrt(g.kappa * R.p * R.inv_rho);
     const Real lmax = a > b ? a : b;
 #pragma unroll
-    for (int h = 0; h < 5; ++h) sf[h] = Real(0.5) * (FL[h] + FR[h] + lmax * (uL[h] - uR[h]));
+    for (int h = 0; h < 5; ++h) sf[h] = Real(0.5) * (FL[h] + FR[h] + lmax * (L.u[h] - R.u[h]));
 }
 
 // Hybrid HLLC/HLLE with Einfeldt-Roe speeds bounded by beta*c and the
 // pressure-jump "smooth" blend: equations/euler/riemann/hllc/riemann_mod.f90:7-104
 template <int D, typename Real>
-__device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const Real* U_L, const Real* U_R, Real* F) {
+__device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState<Real>& L, const NodeState<Real>& R,
+                                             Real* F) {
     constexpr int MN = Mom<D, 0>::idx, MT1 = Mom<D, 1>::idx, MT2 = Mom<D, 2>::idx;
-    const Real kappa = g.kappa;
-    const Real V_L = U_L[MN] / U_L[DENS];
-    const Real V_R = U_R[MN] / U_R[DENS];
-    const Real p_L = pressure_dir<D>(g, U_L);
-    const Real p_R = pressure_dir<D>(g, U_R);
-    const Real c_L = sqrt(kappa * p_L / U_L[DENS]);
-    const Real c_R = sqrt(kappa * p_R / U_R[DENS]);
-    const Real H_L = (U_L[ENER] + p_L) / U_L[DENS];
-    const Real H_R = (U_R[ENER] + p_R) / U_R[DENS];
-    const Real sL = sqrt(U_L[DENS]), sR = sqrt(U_R[DENS]);
-    const Real HTilde = (sL * H_L + sR * H_R) / (sL + sR);
-    const Real vTilde = (sL * V_L + sR * V_R) / (sL + sR);
+    const Real V_L = L.v[D], V_R = R.v[D];
+    const Real p_L = L.p, p_R = R.p;
+    const Real c_L = sqrt(g.kappa * p_L * L.inv_rho);
+    const Real c_R = sqrt(g.kappa * p_R * R.inv_rho);
+    const Real H_L = (L.u[ENER] + p_L) * L.inv_rho;
+    const Real H_R = (R.u[ENER] + p_R) * R.inv_rho;
+    const Real sL = sqrt(L.u[DENS]), sR = sqrt(R.u[DENS]);
+    const Real inv_s = Real(1) / (sL + sR);
+    const Real HTilde = (sL * H_L + sR * H_R) * inv_s;
+    const Real vTilde = (sL * V_L + sR * V_R) * inv_s;
     const Real aTilde = sqrt(g.kappa_m1 * (HTilde - Real(0.5) * (vTilde * vTilde)));
     Real S_L = vTilde - aTilde, t = V_L - g.hllc_beta * c_L;
     S_L = S_L < t ? S_L : t;
@@ -120,86 +160,74 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const Real* U_L
     S_R = S_R > t ? S_R : t;
     S_R = S_R > Real(0) ? S_R : Real(0);
     if (S_L >= Real(0)) {
-        flux<D>(g, U_L, F);
+        flux_s<D>(L, F);
     } else if (S_R <= Real(0)) {
-        flux<D>(g, U_R, F);
+        flux_s<D>(R, F);
     } else {
         Real F_L[5], F_R[5];
-        flux<D>(g, U_L, F_L);
-        flux<D>(g, U_R, F_R);
+        flux_s<D>(L, F_L);
+        flux_s<D>(R, F_R);
         const Real smooth = sqrt(fabs(p_L - p_R) / (p_L + p_R));
         const Real inv = Real(1) / (S_R - S_L);
-        const Real SStar = (p_R - p_L + U_L[MN] * (S_L - V_L) - U_R[MN] * (S_R - V_R)) /
-                           (U_L[DENS] * (S_L - V_L) - U_R[DENS] * (S_R - V_R));
+        const Real SStar = (p_R - p_L + L.u[MN] * (S_L - V_L) - R.u[MN] * (S_R - V_R)) /
+                           (L.u[DENS] * (S_L - V_L) - R.u[DENS] * (S_R - V_R));
         const bool left = (S_L <= Real(0)) && (Real(0) <= SStar);
-        const Real* Us = left ? U_L : U_R;
+        const NodeState<Real>& X = left ? L : R;
         const Real* Fs = left ? F_L : F_R;
         const Real Ss = left ? S_L : S_R;
-        const Real Vs = left ? V_L : V_R;
-        const Real ps = left ? p_L : p_R;
-        const Real QStar = Us[DENS] * (Ss - Vs) / (Ss - SStar);
+        const Real Vs = X.v[D];
+        const Real QStar = X.u[DENS] * (Ss - Vs) / (Ss - SStar);
         Real UStar[5];
         UStar[DENS] = QStar;
         UStar[MN] = QStar * SStar;
-        UStar[MT1] = QStar * Us[MT1] / Us[DENS];
-        UStar[MT2] = QStar * Us[MT2] / Us[DENS];
-        UStar[ENER] = QStar * (Us[ENER] / Us[DENS] + (SStar - Vs) * (SStar + ps / Us[DENS] / (Ss - Vs)));
+        UStar[MT1] = QStar * X.v[MT1 - 1];
+        UStar[MT2] = QStar * X.v[MT2 - 1];
+        UStar[ENER] = QStar * (X.u[ENER] * X.inv_rho + (SStar - Vs) * (SStar + X.p * X.inv_rho / (Ss - Vs)));
         const Real SLSR = S_L * S_R;
 #pragma unroll
         for (int h = 0; h < 5; ++h) {
-            const Real hlle = (S_R * F_L[h] - S_L * F_R[h] + SLSR * (U_R[h] - U_L[h])) * inv;
-            F[h] = smooth * hlle + (Real(1) - smooth) * (Fs[h] + Ss * (UStar[h] - Us[h]));
+            const Real hlle = (S_R * F_L[h] - S_L * F_R[h] + SLSR * (R.u[h] - L.u[h])) * inv;
+            F[h] = smooth * hlle + (Real(1) - smooth) * (Fs[h] + Ss * (UStar[h] - X.u[h]));
         }
     }
 }
 
-// logmean: equations/euler/two_point/chandrashekar/two_point_flux_mod.f90:13-32
-// (series branch restated as written, including the sign of its u^2 term).
+// logmean, equations/euler/two_point/chandrashekar/two_point_flux_mod.f90:
+// 13-32 (series branch as written, including the sign of its u^2 term),
+// returned together with its reciprocal so that the flux needs no further
+// division by it: 1/L = log(a/b)/(a-b) on the log branch.
 template <typename Real>
-__device__ __forceinline__ Real logmean(Real a, Real b) {
+__device__ __forceinline__ void logmean2(Real a, Real b, Real& lm, Real& inv_lm) {
     const Real x = a / b;
     if (fabs(a - b) < Real(1e-4)) {
         const Real c1 = Real(1.0 / 6.0), c2 = Real(2.0 / 45.0), c3 = Real(22.0 / 945.0);
         const Real u = (x * (x - Real(2)) + Real(1)) / (x * (x + Real(2)) + Real(1));
-        return (a + b) * (Real(0.5) - u * (c1 - u * (c2 - c3 * u)));
+        lm = (a + b) * (Real(0.5) - u * (c1 - u * (c2 - c3 * u)));
+        inv_lm = Real(1) / lm;
+    } else {
+        const Real l = log(x);
+        lm = (a - b) / l;
+        inv_lm = l / (a - b);
     }
-    return (a - b) / log(x);
-}
-
-// Per-node quantities of the two-point flux that depend on one state only
-// (velocity, pressure, beta); computed once per node instead of once per
-// pair.  Same expressions as two_point_flux_mod.f90:49-61.
-template <typename Real>
-struct NodePrim {
-    Real rho, v[3], p, beta;
-};
-
-template <typename Real>
-__device__ __forceinline__ NodePrim<Real> node_prim(const Gas<Real>& g, const Real* u) {
-    NodePrim<Real> q;
-    q.rho = u[DENS];
-    q.p = pressure(g, u);
-    q.beta = Real(0.5) * u[DENS] / q.p;
-    q.v[0] = u[MOMX] / u[DENS];
-    q.v[1] = u[MOMY] / u[DENS];
-    q.v[2] = u[MOMZ] / u[DENS];
-    return q;
 }
 
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:
 // two_point_flux_mod.f90:34-77, rotated to direction D (:79-141).
 template <int D, typename Real>
-__device__ __forceinline__ void ec_flux(const Gas<Real>& g, const NodePrim<Real>& a, const NodePrim<Real>& b, Real* f) {
+__device__ __forceinline__ void ec_flux(const Gas<Real>& g, const NodeState<Real>& a, const NodeState<Real>& b,
+                                        Real* f) {
     constexpr int N0 = D, N1 = (D + 1) % 3, N2 = (D + 2) % 3;
-    const Real dens = logmean(a.rho, b.rho);
+    Real dens, inv_dens, lnb, inv_lnb;
+    logmean2(a.u[DENS], b.u[DENS], dens, inv_dens);
+    logmean2(a.beta, b.beta, lnb, inv_lnb);
     const Real velx = Real(0.5) * (a.v[N0] + b.v[N0]);
     const Real vely = Real(0.5) * (a.v[N1] + b.v[N1]);
     const Real velz = Real(0.5) * (a.v[N2] + b.v[N2]);
-    const Real pres = Real(0.5) * (a.rho + b.rho) / (a.beta + b.beta);
+    const Real pres = Real(0.5) * (a.u[DENS] + b.u[DENS]) / (a.beta + b.beta);
     const Real ssum = (a.v[N0] * a.v[N0] + b.v[N0] * b.v[N0]) + (a.v[N1] * a.v[N1] + b.v[N1] * b.v[N1]) +
                       (a.v[N2] * a.v[N2] + b.v[N2] * b.v[N2]);
-    const Real enth = g.half_inv_km1 / logmean(a.beta, b.beta) + pres / dens + velx * velx + vely * vely +
-                      velz * velz - Real(0.25) * ssum;
+    const Real enth = g.half_inv_km1 * inv_lnb + pres * inv_dens + velx * velx + vely * vely + velz * velz -
+                      Real(0.25) * ssum;
     const Real fr = dens * velx;
     f[DENS] = fr;
     f[1 + N0] = fr * velx + pres;
@@ -210,10 +238,10 @@ __device__ __forceinline__ void ec_flux(const Gas<Real>& g, const NodePrim<Real>
 
 // Central two-point flux: equations/euler/two_point/standard/two_point_flux_mod.f90:13-23
 template <int D, typename Real>
-__device__ __forceinline__ void central_flux(const Gas<Real>& g, const Real* u, const Real* v, Real* r) {
+__device__ __forceinline__ void central_flux(const NodeState<Real>& a, const NodeState<Real>& b, Real* r) {
     Real fu[5], fv[5];
-    flux<D>(g, u, fu);
-    flux<D>(g, v, fv);
+    flux_s<D>(a, fu);
+    flux_s<D>(b, fv);
 #pragma unroll
     for (int h = 0; h < 5; ++h) r[h] = Real(0.5) * (fu[h] + fv[h]);
 }

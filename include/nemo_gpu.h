@@ -67,7 +67,9 @@ typedef struct nemo_gpu_config {
     int32_t indicator;     /* NEMO_IND_* */
     int32_t timedisc;      /* NEMO_TD_* */
     int32_t repair;        /* 1: kernel_repair after every stage */
-    int32_t precision;     /* 0: FP64 operator */
+    int32_t precision;     /* 0: FP64 operator; 1: FP32 operator (fluxes, Riemann
+                              solvers, indicator, accumulation in FP32; state,
+                              RK registers, CFL and repair in FP64) */
     double gamma;          /* setup_config_mod::kappa */
     double alpha_max;      /* blending factor clamp */
     double alpha_min;      /* blending factors below this become 0 */

@@ -51,6 +51,7 @@ struct OpTable {
 struct RKCoeffs {
     double lsrk_a[5], lsrk_c[5];
     double ssp[23];  // A0..A3, B0..B3, C0..C3, D0..D3, E0..E6
+    double w64[8];   // FP64 quadrature weights (repair means)
 };
 
 template <int M, typename Real>
@@ -74,6 +75,7 @@ struct StageArgs {
     int repair;
     double alpha_max, alpha_min, ind_threshold, ind_s;
     Gas<Real> gas;
+    Gas<double> gas64;  // FP64 constants (CFL, FP64 pressures of the FP32 operator)
     OpTable<M, Real> op;
     RKCoeffs rk;
 };
@@ -114,8 +116,9 @@ struct StageShape {
 // states once; the surface Riemann problems, the M(M-1)/2 two-point fluxes
 // and the M-1 FV interface fluxes all read those states.
 template <int D, int M, int BE, int VOL, int SURF, typename Real>
-__device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, Real (*s_u)[BE * M * M * M],
-                                               Real (*s_acc)[BE * M * M * M], const Real* s_alpha) {
+__device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, double (*s_u)[BE * M * M * M],
+                                               Real (*s_acc)[BE * M * M * M], const Real* s_alpha,
+                                               const double* s_p64) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
     constexpr bool BETA = (VOL == VOL_EC);
@@ -129,12 +132,15 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     const Gas<Real>& g = a.gas;
     const int base = el * M3;
 
+    constexpr bool F32 = sizeof(Real) == 4;
     NodeState<Real> S[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
+        const int n = base + nidx<D, M>(i, ta, tb);
 #pragma unroll
-        for (int h = 0; h < 5; ++h) S[i].u[h] = s_u[h][base + nidx<D, M>(i, ta, tb)];
+        for (int h = 0; h < 5; ++h) S[i].u[h] = Real(s_u[h][n]);
         make_state<D, BETA>(g, S[i]);
+        if constexpr (F32) S[i].p64 = s_p64[n];
     }
     Real jac = Real(0), sdx = Real(0);
     Real fm[5], fp[5];
@@ -148,10 +154,17 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
             sdx = Real(double(M) / dx);
             const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
             const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
+            double dm[5], dp[5];
 #pragma unroll
             for (int h = 0; h < 5; ++h) {
-                Nm.u[h] = Real(um[h * EM3]);
-                Np.u[h] = Real(up[h * EM3]);
+                dm[h] = um[h * EM3];
+                dp[h] = up[h * EM3];
+                Nm.u[h] = Real(dm[h]);
+                Np.u[h] = Real(dp[h]);
+            }
+            if constexpr (F32) {
+                Nm.p64 = pressure(a.gas64, dm);
+                Np.p64 = pressure(a.gas64, dp);
             }
         } else {
 #pragma unroll
@@ -159,6 +172,8 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
                 Nm.u[h] = S[0].u[h];
                 Np.u[h] = S[M - 1].u[h];
             }
+            Nm.p64 = S[0].p64;
+            Np.p64 = S[M - 1].p64;
         }
         make_state<D, false>(g, Nm);
         make_state<D, false>(g, Np);
@@ -268,8 +283,8 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
 // 274-311, 584-601), alpha from the logistic ramp around the threshold
 // 0.5*10^(-1.8*M^0.25), clipped by alpha_min/alpha_max.
 template <int M, int BE, typename Real>
-__device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long long e0, Real (*s_u)[BE * M * M * M],
-                                             Real (*s_acc)[BE * M * M * M], Real* s_alpha) {
+__device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long long e0, double (*s_u)[BE * M * M * M],
+                                             Real (*s_acc)[BE * M * M * M], Real* s_alpha, const double* s_p64) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
     const int tid = threadIdx.x;
@@ -277,10 +292,14 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
     Real* t1 = s_acc[0];
     Real* t2 = s_acc[1];
     for (int idx = tid; idx < NN; idx += NT) {
-        Real uu[5];
+        if constexpr (sizeof(Real) == 4) {
+            q[idx] = Real(s_u[DENS][idx] * s_p64[idx]);
+        } else {
+            double uu[5];
 #pragma unroll
-        for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
-        q[idx] = uu[DENS] * pressure(a.gas, uu);
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
+            q[idx] = uu[DENS] * pressure(a.gas64, uu);
+        }
     }
     __syncthreads();
     const int el = tid / M2, line = tid - el * M2;
@@ -474,9 +493,10 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
 // the M nodes of x-line t, then one thread per element folds its lines.
 template <int M, int BE, int R>
 __device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __restrict__ delta, long long e0,
-                                          long long nelem, double (*s_u)[BE * M * M * M], double (*s_w)[R],
+                                          long long nelem, double (*s_u)[BE * M * M * M], double* s_wbase,
                                           double* __restrict__ out) {
     static_assert(R >= BE * M * M, "work rows must hold one value per thread");
+    double (*s_w)[R] = reinterpret_cast<double (*)[R]>(s_wbase);
     constexpr int M2 = M * M, M3 = M2 * M;
     const int tid = threadIdx.x;
     const int el = tid / M2, line = tid - el * M2;
@@ -540,13 +560,14 @@ __device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __
     }
 }
 
-template <int M, int BE, int VOL, int SURF>
-__global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> a) {
-    using Real = double;
+template <int M, int BE, int VOL, int SURF, typename Real>
+__global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a) {
     using Sh = StageShape<M, BE>;
     constexpr int M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
-    __shared__ Real s_u[5][NN];
-    __shared__ Real s_acc[5][NN];
+    constexpr bool F32 = sizeof(Real) == 4;
+    __shared__ double s_u[5][NN];          // FP64 state: RK update, repair, CFL
+    __shared__ Real s_acc[5][NN];          // RHS accumulators in operator precision
+    __shared__ double s_p64[F32 ? NN : 1]; // FP32 operator: FP64 pressure per node
     __shared__ Real s_alpha[BE];
     __shared__ int s_bad[BE];
 
@@ -567,22 +588,31 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
         s_bad[tid] = 0;
     }
     __syncthreads();
+    if constexpr (F32) {
+        for (int idx = tid; idx < NN; idx += NT) {
+            double uu[5];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
+            s_p64[idx] = pressure(a.gas64, uu);
+        }
+        __syncthreads();
+    }
 
-    if (a.indicator == IND_PP) indicator_pp<M, BE, Real>(a, e0, s_u, s_acc, s_alpha);
+    if (a.indicator == IND_PP) indicator_pp<M, BE, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
 
     // ---- three direction passes ----------------------------------------
-    direction_pass<0, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
+    direction_pass<0, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
     __syncthreads();
-    direction_pass<1, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
+    direction_pass<1, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
     __syncthreads();
-    direction_pass<2, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha);
+    direction_pass<2, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
     __syncthreads();
 
     // ---- RHS only (parity path) -----------------------------------------
     if (a.rhs_out) {
 #pragma unroll
         for (int h = 0; h < 5; ++h)
-            for (int idx = tid; idx < nvalid_nodes; idx += NT) a.rhs_out[h * EM3 + e0 * M3 + idx] = s_acc[h][idx];
+            for (int idx = tid; idx < nvalid_nodes; idx += NT) a.rhs_out[h * EM3 + e0 * M3 + idx] = double(s_acc[h][idx]);
         return;
     }
 
@@ -594,7 +624,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
     for (int h = 0; h < 5; ++h)
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             const long long gi = h * EM3 + e0 * M3 + idx;
-            const double rhs = s_acc[h][idx];
+            const double rhs = double(s_acc[h][idx]);
             const double u = s_u[h][idx];
             double st;
             if (a.timedisc == TD_LSRK54) {
@@ -621,15 +651,15 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
     // ---- positivity repair (kernel_repair after the probe) ---------------
     if (a.repair) {
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
-            Real uu[5];
+            double uu[5];
 #pragma unroll
             for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
-            if (!isvalid(a.gas, uu)) s_bad[idx / M3] = 1;
+            if (!isvalid(a.gas64, uu)) s_bad[idx / M3] = 1;
         }
         __syncthreads();
         if (tid < BE && s_bad[tid]) {
-            Real* su[5] = {s_u[0], s_u[1], s_u[2], s_u[3], s_u[4]};
-            if (!repair_element<M, Real>(a.gas, a.op.w, su, tid * M3)) atomicOr(a.status, 1);
+            double* su[5] = {s_u[0], s_u[1], s_u[2], s_u[3], s_u[4]};
+            if (!repair_element<M, double>(a.gas64, a.rk.w64, su, tid * M3)) atomicOr(a.status, 1);
         }
         __syncthreads();
     }
@@ -648,7 +678,9 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, double> 
         }
 
     // ---- CFL partials of the new state (last stage of a step) -----------
-    if (a.cfl_part) block_cfl<M, BE, NN>(a.gas, a.delta, e0, a.nelem, s_u, s_acc, a.cfl_part + 4 * blk);
+    if (a.cfl_part)
+        block_cfl<M, BE, F32 ? NN / 2 : NN>(a.gas64, a.delta, e0, a.nelem, s_u, reinterpret_cast<double*>(&s_acc[0][0]),
+                                          a.cfl_part + 4 * blk);
 }
 
 // Standalone CFL reduction of a state in HBM (first step, calc_timestep):
@@ -669,7 +701,7 @@ __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restric
         for (int idx = tid; idx < NN; idx += NT)
             s_u[h][idx] = idx < nvalid ? u[h * EM3 + e0 * M3 + idx] : (h == 0 || h == 4 ? 1.0 : 0.0);
     __syncthreads();
-    block_cfl<M, BE, NT>(g, delta, e0, nelem, s_u, s_w, part + 4 * blk);
+    block_cfl<M, BE, NT>(g, delta, e0, nelem, s_u, &s_w[0][0], part + 4 * blk);
 }
 
 // Deterministic reduction of the per-block partials (min and max are exact,

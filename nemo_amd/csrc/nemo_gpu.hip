@@ -216,6 +216,7 @@ nemo::RKCoeffs make_rk() {
         rk.lsrk_c[i] = c[i];
     }
     for (int i = 0; i < 23; ++i) rk.ssp[i] = s[i];
+    for (int i = 0; i < 8; ++i) rk.w64[i] = 0.0;
     return rk;
 }
 
@@ -277,10 +278,20 @@ struct Tile<4> { static constexpr int BE = 4; };
 template <>
 struct Tile<5> { static constexpr int BE = 2; };
 
-template <int M>
-nemo::StageArgs<M, double> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
-                                     double* alpha_out, double* cfl_part) {
-    nemo::StageArgs<M, double> a{};
+template <typename Real>
+nemo::Gas<Real> make_gas(double k) {
+    nemo::Gas<Real> g;
+    g.kappa = Real(k);
+    g.kappa_m1 = Real(k - 1.0);
+    g.hllc_beta = Real(std::sqrt(0.5 * (k - 1) / k));
+    g.half_inv_km1 = Real(0.5 / (k - 1.0));
+    return g;
+}
+
+template <int M, typename Real>
+nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
+                                   double* alpha_out, double* cfl_part) {
+    nemo::StageArgs<M, Real> a{};
     a.u_in = s.u[s.cur];
     a.u_out = s.u[s.cur ^ 1];
     a.r0 = s.r0;
@@ -302,32 +313,31 @@ nemo::StageArgs<M, double> make_args(Solver& s, int stage, double dt, const doub
     a.alpha_min = s.cfg.alpha_min;
     a.ind_threshold = 0.5 * std::pow(10.0, -1.8 * std::pow(double(M), 0.25));
     a.ind_s = std::log((1.0 - 0.0001) / 0.0001);
-    const double k = s.cfg.gamma;
-    a.gas.kappa = k;
-    a.gas.kappa_m1 = k - 1.0;
-    a.gas.hllc_beta = std::sqrt(0.5 * (k - 1) / k);
-    a.gas.half_inv_km1 = 0.5 / (k - 1.0);
-    a.op = make_table<M, double>(s.ops);
+    a.gas = make_gas<Real>(s.cfg.gamma);
+    a.gas64 = make_gas<double>(s.cfg.gamma);
+    a.op = make_table<M, Real>(s.ops);
     a.rk = s.rk;
+    for (int i = 0; i < M; ++i) a.rk.w64[i] = s.ops.w[i];
     return a;
 }
 
-template <int M, int VOL, int SURF>
-void launch_stage_t(Solver& s, const nemo::StageArgs<M, double>& a) {
+template <int M, int VOL, int SURF, typename Real>
+void launch_stage_t(Solver& s, const nemo::StageArgs<M, Real>& a) {
     constexpr int BE = Tile<M>::BE;
     using Sh = nemo::StageShape<M, BE>;
     const long long nblocks = (s.nelem + BE - 1) / BE;
-    hipLaunchKernelGGL((nemo::stage_kernel<M, BE, VOL, SURF>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0, s.stream, a);
+    hipLaunchKernelGGL((nemo::stage_kernel<M, BE, VOL, SURF, Real>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0, s.stream,
+                       a);
 }
 
-template <int M>
-int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                   double* cfl_part) {
-    const auto a = make_args<M>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
+template <int M, typename Real>
+int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
+                    double* cfl_part) {
+    const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
 #define NEMO_CASE(V_, S_)                              \
     if (vol == V_ && surf == S_) {                     \
-        launch_stage_t<M, V_, S_>(s, a);               \
+        launch_stage_t<M, V_, S_, Real>(s, a);         \
         return NEMO_OK;                                \
     }
     NEMO_CASE(nemo::VOL_WEAK, nemo::SURF_RUSANOV)
@@ -338,6 +348,14 @@ int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double
     NEMO_CASE(nemo::VOL_EC, nemo::SURF_HLLC)
 #undef NEMO_CASE
     return fail(NEMO_ERR_ARG, "unsupported volume/surface combination");
+}
+
+template <int M>
+int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
+                   double* cfl_part) {
+    if (s.cfg.precision == 1)
+        return launch_stage_mr<M, float>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
+    return launch_stage_mr<M, double>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
 }
 
 int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
@@ -384,12 +402,7 @@ long long nblocks_for(const Solver& s) {
 template <int M>
 void launch_cfl_m(Solver& s) {
     constexpr int BE = Tile<M>::BE;
-    nemo::Gas<double> g;
-    const double k = s.cfg.gamma;
-    g.kappa = k;
-    g.kappa_m1 = k - 1.0;
-    g.hllc_beta = std::sqrt(0.5 * (k - 1) / k);
-    g.half_inv_km1 = 0.5 / (k - 1.0);
+    const nemo::Gas<double> g = make_gas<double>(s.cfg.gamma);
     hipLaunchKernelGGL((nemo::cfl_kernel<M, BE>), dim3(unsigned(stage_blocks<M>(s))), dim3(BE * M * M), 0,
                        s.stream, s.u[s.cur], s.delta, s.nelem, g, s.part);
 }
@@ -482,7 +495,7 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
     if (cfg->surface < 0 || cfg->surface > 1) return fail(NEMO_ERR_ARG, "invalid surface flux");
     if (cfg->indicator < 0 || cfg->indicator > 1) return fail(NEMO_ERR_ARG, "invalid indicator");
     if (cfg->timedisc < 0 || cfg->timedisc > 1) return fail(NEMO_ERR_ARG, "invalid timedisc");
-    if (cfg->precision != 0) return fail(NEMO_ERR_ARG, "precision: only the FP64 operator is built");
+    if (cfg->precision < 0 || cfg->precision > 1) return fail(NEMO_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 operator)");
     if (!(cfg->gamma > 1.0)) return fail(NEMO_ERR_ARG, "gamma must exceed 1");
     Solver* s = new Solver();
     s->cfg = *cfg;

@@ -34,6 +34,19 @@ struct Gas {
     Real half_inv_km1; // 0.5/(kappa-1)
 };
 
+// Arithmetic by operator precision.  FP64 keeps IEEE division, sqrt and log
+// (the plain path matches the reference to round-off); the FP32 operator
+// uses the hardware reciprocal, square root and log2 (v_rcp_f32, v_sqrt_f32,
+// v_log_f32, about 1 ulp), which is what makes it cheap.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double frcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ float frcp(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double flog(double a) { return log(a); }
+__device__ __forceinline__ float flog(float a) { return __builtin_amdgcn_logf(a) * 0.69314718055994530942f; }
+
 // pressure_elemental: equations/euler/equations_mod.f90:131-140.
 // Evaluated without FMA contraction and, for a directional call, with the
 // momentum squares summed in the order of the state rotated to that
@@ -97,16 +110,22 @@ struct NodeState {
     Real v[3];   // velocity components x, y, z
     Real p;      // pressure, rotated order of the pass direction
     Real beta;   // 0.5*rho/p (two-point flux only)
+    double p64;  // FP32 operator only: the FP64 pressure, for the HLLC/HLLE blend
 };
 
 template <int D, bool BETA, typename Real>
 __device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& s) {
-    s.inv_rho = Real(1) / s.u[DENS];
+    s.inv_rho = frcp(s.u[DENS]);
     s.v[0] = s.u[MOMX] * s.inv_rho;
     s.v[1] = s.u[MOMY] * s.inv_rho;
     s.v[2] = s.u[MOMZ] * s.inv_rho;
-    s.p = pressure_dir<D>(g, s.u);
-    if constexpr (BETA) s.beta = Real(0.5) * s.u[DENS] / s.p;
+    if constexpr (sizeof(Real) == 8) {
+        s.p = pressure_dir<D>(g, s.u);
+    } else {
+        s.p = g.kappa_m1 * (s.u[ENER] - Real(0.5) * s.inv_rho *
+                                            (s.u[MOMX] * s.u[MOMX] + s.u[MOMY] * s.u[MOMY] + s.u[MOMZ] * s.u[MOMZ]));
+    }
+    if constexpr (BETA) s.beta = fdiv(Real(0.5) * s.u[DENS], s.p);
 }
 
 // xflux (equations_mod.f90:64-79) from a node state.
@@ -128,8 +147,8 @@ __device__ __forceinline__ void riemann_rusanov(const Gas<Real>& g, const NodeSt
     Real FL[5], FR[5];
     flux_s<D>(L, FL);
     flux_s<D>(R, FR);
-    const Real a = fabs(L.v[D]) + sqrt(g.kappa * L.p * L.inv_rho);
-    const Real b = fabs(R.v[D]) + sqrt(g.kappa * R.p * R.inv_rho);
+    const Real a = fabs(L.v[D]) + fsqrt(g.kappa * L.p * L.inv_rho);
+    const Real b = fabs(R.v[D]) + fsqrt(g.kappa * R.p * R.inv_rho);
     const Real lmax = a > b ? a : b;
 #pragma unroll
     for (int h = 0; h < 5; ++h) sf[h] = Real(0.5) * (FL[h] + FR[h] + lmax * (L.u[h] - R.u[h]));
@@ -143,15 +162,15 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
     constexpr int MN = Mom<D, 0>::idx, MT1 = Mom<D, 1>::idx, MT2 = Mom<D, 2>::idx;
     const Real V_L = L.v[D], V_R = R.v[D];
     const Real p_L = L.p, p_R = R.p;
-    const Real c_L = sqrt(g.kappa * p_L * L.inv_rho);
-    const Real c_R = sqrt(g.kappa * p_R * R.inv_rho);
+    const Real c_L = fsqrt(g.kappa * p_L * L.inv_rho);
+    const Real c_R = fsqrt(g.kappa * p_R * R.inv_rho);
     const Real H_L = (L.u[ENER] + p_L) * L.inv_rho;
     const Real H_R = (R.u[ENER] + p_R) * R.inv_rho;
-    const Real sL = sqrt(L.u[DENS]), sR = sqrt(R.u[DENS]);
-    const Real inv_s = Real(1) / (sL + sR);
+    const Real sL = fsqrt(L.u[DENS]), sR = fsqrt(R.u[DENS]);
+    const Real inv_s = frcp(sL + sR);
     const Real HTilde = (sL * H_L + sR * H_R) * inv_s;
     const Real vTilde = (sL * V_L + sR * V_R) * inv_s;
-    const Real aTilde = sqrt(g.kappa_m1 * (HTilde - Real(0.5) * (vTilde * vTilde)));
+    const Real aTilde = fsqrt(g.kappa_m1 * (HTilde - Real(0.5) * (vTilde * vTilde)));
     Real S_L = vTilde - aTilde, t = V_L - g.hllc_beta * c_L;
     S_L = S_L < t ? S_L : t;
     S_L = S_L < Real(0) ? S_L : Real(0);
@@ -167,22 +186,31 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
         Real F_L[5], F_R[5];
         flux_s<D>(L, F_L);
         flux_s<D>(R, F_R);
-        const Real smooth = sqrt(fabs(p_L - p_R) / (p_L + p_R));
-        const Real inv = Real(1) / (S_R - S_L);
-        const Real SStar = (p_R - p_L + L.u[MN] * (S_L - V_L) - R.u[MN] * (S_R - V_R)) /
-                           (L.u[DENS] * (S_L - V_L) - R.u[DENS] * (S_R - V_R));
+        Real smooth;
+        if constexpr (sizeof(Real) == 8) {
+            smooth = sqrt(fabs(p_L - p_R) / (p_L + p_R));
+        } else {
+            // The blend weight sqrt(|dp|/sum p) turns a relative pressure
+            // error e into a weight of about sqrt(e): FP32 pressures would
+            // give ~2e-4 instead of the FP64 ~1e-8 on uniform-pressure
+            // contacts, so the FP32 operator takes dp from FP64 pressures.
+            smooth = fsqrt(float(fabs(L.p64 - R.p64)) * frcp(float(L.p64 + R.p64)));
+        }
+        const Real inv = frcp(S_R - S_L);
+        const Real SStar = fdiv(p_R - p_L + L.u[MN] * (S_L - V_L) - R.u[MN] * (S_R - V_R),
+                                L.u[DENS] * (S_L - V_L) - R.u[DENS] * (S_R - V_R));
         const bool left = (S_L <= Real(0)) && (Real(0) <= SStar);
         const NodeState<Real>& X = left ? L : R;
         const Real* Fs = left ? F_L : F_R;
         const Real Ss = left ? S_L : S_R;
         const Real Vs = X.v[D];
-        const Real QStar = X.u[DENS] * (Ss - Vs) / (Ss - SStar);
+        const Real QStar = fdiv(X.u[DENS] * (Ss - Vs), Ss - SStar);
         Real UStar[5];
         UStar[DENS] = QStar;
         UStar[MN] = QStar * SStar;
         UStar[MT1] = QStar * X.v[MT1 - 1];
         UStar[MT2] = QStar * X.v[MT2 - 1];
-        UStar[ENER] = QStar * (X.u[ENER] * X.inv_rho + (SStar - Vs) * (SStar + X.p * X.inv_rho / (Ss - Vs)));
+        UStar[ENER] = QStar * (X.u[ENER] * X.inv_rho + (SStar - Vs) * (SStar + fdiv(X.p * X.inv_rho, Ss - Vs)));
         const Real SLSR = S_L * S_R;
 #pragma unroll
         for (int h = 0; h < 5; ++h) {
@@ -211,6 +239,28 @@ __device__ __forceinline__ void logmean2(Real a, Real b, Real& lm, Real& inv_lm)
     }
 }
 
+// FP32 log-mean.  The reference's log branch loses ~eps/|a/b-1| relative
+// accuracy near its switch |a-b| = 1e-4 (harmless in FP64, 6e-4 in FP32), so
+// the FP32 operator uses the series of Ismail & Roe (JCP 228, 2009) in
+// f = (a-b)/(a+b):  L = (a+b) / (2 sum_k f^2k/(2k+1)), eight terms for
+// f^2 < 0.1 (truncation < 1e-9), and the log form beyond, where
+// |log(a/b)| > 0.65.
+__device__ __forceinline__ void logmean2(float a, float b, float& lm, float& inv_lm) {
+    const float r = __builtin_amdgcn_rcpf(a + b);
+    const float f = (a - b) * r;
+    const float u = f * f;
+    if (u < 0.1f) {
+        const float F = 1.0f + u * (1.0f / 3.0f + u * (1.0f / 5.0f + u * (1.0f / 7.0f + u * (1.0f / 9.0f +
+                        u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f)))))));
+        lm = 0.5f * (a + b) * __builtin_amdgcn_rcpf(F);
+        inv_lm = 2.0f * F * r;
+    } else {
+        const float l = flog(a * __builtin_amdgcn_rcpf(b));
+        lm = (a - b) * __builtin_amdgcn_rcpf(l);
+        inv_lm = l * __builtin_amdgcn_rcpf(a - b);
+    }
+}
+
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:
 // two_point_flux_mod.f90:34-77, rotated to direction D (:79-141).
 template <int D, typename Real>
@@ -223,7 +273,7 @@ __device__ __forceinline__ void ec_flux(const Gas<Real>& g, const NodeState<Real
     const Real velx = Real(0.5) * (a.v[N0] + b.v[N0]);
     const Real vely = Real(0.5) * (a.v[N1] + b.v[N1]);
     const Real velz = Real(0.5) * (a.v[N2] + b.v[N2]);
-    const Real pres = Real(0.5) * (a.u[DENS] + b.u[DENS]) / (a.beta + b.beta);
+    const Real pres = fdiv(Real(0.5) * (a.u[DENS] + b.u[DENS]), a.beta + b.beta);
     const Real ssum = (a.v[N0] * a.v[N0] + b.v[N0] * b.v[N0]) + (a.v[N1] * a.v[N1] + b.v[N1] * b.v[N1]) +
                       (a.v[N2] * a.v[N2] + b.v[N2] * b.v[N2]);
     const Real enth = g.half_inv_km1 * inv_lnb + pres * inv_dens + velx * velx + vely * vely + velz * velz -

@@ -1,0 +1,177 @@
+!! ========================================================================== !!
+!! ISO_C_BINDING interfaces to the MI355X spatial operator (include/nemo_gpu.h).
+!!
+!! Modelled on the reference's own C bindings (mesh/p4est/wrapper/
+!! p4wrap_interfaces_mod.f90:46-120): opaque type(c_ptr) handles, plain
+!! arrays, integer status returns.  A Fortran host keeps nemo's driver loop
+!! (driver_mod.f90:62-150) and calls these entry points from its kernel,
+!! timedisc and timestep hooks.
+!! ========================================================================== !!
+module nemo_gpu_mod
+
+use iso_c_binding
+
+implicit none
+
+integer(c_int), parameter :: NEMO_OK            = 0
+integer(c_int), parameter :: NEMO_ERR_ARG       = 1
+integer(c_int), parameter :: NEMO_ERR_HIP       = 2
+integer(c_int), parameter :: NEMO_ERR_STATE     = 3
+integer(c_int), parameter :: NEMO_ERR_NOT_READY = 4
+
+integer(c_int32_t), parameter :: NEMO_EQ_EULER     = 0
+integer(c_int32_t), parameter :: NEMO_VOL_WEAK     = 0
+integer(c_int32_t), parameter :: NEMO_VOL_CENTRAL  = 1
+integer(c_int32_t), parameter :: NEMO_VOL_EC       = 2
+integer(c_int32_t), parameter :: NEMO_SURF_RUSANOV = 0
+integer(c_int32_t), parameter :: NEMO_SURF_HLLC    = 1
+integer(c_int32_t), parameter :: NEMO_IND_NONE     = 0
+integer(c_int32_t), parameter :: NEMO_IND_PP       = 1
+integer(c_int32_t), parameter :: NEMO_TD_LSRK54    = 0
+integer(c_int32_t), parameter :: NEMO_TD_SSPRK54   = 1
+
+!! nemo_gpu_config: the reference's compile-time choices (preproc.py flags,
+!! setup_config_mod constants) as a runtime record.
+type, bind(c) :: nemo_gpu_config
+    integer(c_int32_t) :: n_poly
+    integer(c_int32_t) :: equations
+    integer(c_int32_t) :: volume
+    integer(c_int32_t) :: surface
+    integer(c_int32_t) :: indicator
+    integer(c_int32_t) :: timedisc
+    integer(c_int32_t) :: repair
+    integer(c_int32_t) :: precision
+    real(c_double)     :: gamma
+    real(c_double)     :: alpha_max
+    real(c_double)     :: alpha_min
+    real(c_double)     :: cfl
+    real(c_double)     :: mu0
+end type
+
+interface
+
+    function nemo_gpu_create(cfg, handle) result(rc) bind(c, name='nemo_gpu_create')
+        import :: c_int, c_ptr, nemo_gpu_config
+        type(nemo_gpu_config), intent(in) :: cfg
+        type(c_ptr), intent(out)          :: handle
+        integer(c_int)                    :: rc
+    end function
+
+    function nemo_gpu_destroy(handle) result(rc) bind(c, name='nemo_gpu_destroy')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: handle
+        integer(c_int)     :: rc
+    end function
+
+    function nemo_gpu_upload_mesh(handle, nelem, nbr, delta) result(rc) bind(c, name='nemo_gpu_upload_mesh')
+        import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+        type(c_ptr), value                :: handle
+        integer(c_int64_t), value         :: nelem
+        integer(c_int32_t), intent(in)    :: nbr(*)
+        real(c_double), intent(in)        :: delta(*)
+        integer(c_int)                    :: rc
+    end function
+
+    function nemo_gpu_upload_state(handle, u) result(rc) bind(c, name='nemo_gpu_upload_state')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value         :: handle
+        real(c_double), intent(in) :: u(*)
+        integer(c_int)             :: rc
+    end function
+
+    function nemo_gpu_download_state(handle, u) result(rc) bind(c, name='nemo_gpu_download_state')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value          :: handle
+        real(c_double), intent(out) :: u(*)
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_rhs(handle, rhs) result(rc) bind(c, name='nemo_gpu_rhs')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value          :: handle
+        real(c_double), intent(out) :: rhs(*)
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_indicator(handle, alpha) result(rc) bind(c, name='nemo_gpu_indicator')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value          :: handle
+        real(c_double), intent(out) :: alpha(*)
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_calc_timestep(handle, dt, lmax, vmax) result(rc) bind(c, name='nemo_gpu_calc_timestep')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value          :: handle
+        real(c_double), intent(out) :: dt, lmax, vmax
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_stage(handle, stage, dt) result(rc) bind(c, name='nemo_gpu_stage')
+        import :: c_int, c_ptr, c_int32_t, c_double
+        type(c_ptr), value        :: handle
+        integer(c_int32_t), value :: stage
+        real(c_double), value     :: dt
+        integer(c_int)            :: rc
+    end function
+
+    function nemo_gpu_step(handle, dt) result(rc) bind(c, name='nemo_gpu_step')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value    :: handle
+        real(c_double), value :: dt
+        integer(c_int)        :: rc
+    end function
+
+    function nemo_gpu_run(handle, nsteps, dt_hist) result(rc) bind(c, name='nemo_gpu_run')
+        import :: c_int, c_ptr, c_int32_t
+        type(c_ptr), value        :: handle
+        integer(c_int32_t), value :: nsteps
+        type(c_ptr), value        :: dt_hist
+        integer(c_int)            :: rc
+    end function
+
+    function nemo_gpu_synchronize(handle) result(rc) bind(c, name='nemo_gpu_synchronize')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: handle
+        integer(c_int)     :: rc
+    end function
+
+    function nemo_gpu_get_status(handle, code) result(rc) bind(c, name='nemo_gpu_get_status')
+        import :: c_int, c_ptr, c_int32_t
+        type(c_ptr), value              :: handle
+        integer(c_int32_t), intent(out) :: code
+        integer(c_int)                  :: rc
+    end function
+
+    function nemo_gpu_operators(n_poly, x, w, d, vinv) result(rc) bind(c, name='nemo_gpu_operators')
+        import :: c_int, c_int32_t, c_double
+        integer(c_int32_t), value   :: n_poly
+        real(c_double), intent(out) :: x(*), w(*), d(*), vinv(*)
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_error_message(buf, len) result(rc) bind(c, name='nemo_gpu_error_message')
+        import :: c_int, c_int32_t, c_char
+        character(kind=c_char), intent(out) :: buf(*)
+        integer(c_int32_t), value           :: len
+        integer(c_int)                      :: rc
+    end function
+
+end interface
+
+contains
+
+!! Last error message as a Fortran string (for raise_error).
+function nemo_gpu_last_error() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char) :: buf(512)
+    integer :: i, rc
+    rc = nemo_gpu_error_message(buf, 512_c_int32_t)
+    msg = ''
+    do i = 1, 512
+        if (buf(i) == c_null_char) exit
+        msg(i:i) = buf(i)
+    end do
+end function
+
+end module

@@ -1,0 +1,71 @@
+"""The Fortran host driver (nemo_amd/fortran) over the C-ABI.
+
+GPU: the driver runs the reference's loop (calc_timestep, five stages per
+step, status check) through the ISO_C_BINDING interfaces; its dumped
+initial state is advanced by the oracle for the same number of steps and
+compared with the dumped final state (and the time steps).
+CPU: without a device the driver reports the C-ABI error and stops with a
+nonzero code (the reference's raise_error/rt%stat convention).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nemo_amd import mesh as meshmod
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "nemo_amd", "nemo_gpu_driver")
+
+
+def _read_state(path):
+    raw = open(path, "rb").read()
+    nelem, n_poly, nvars = np.frombuffer(raw[:24], dtype=np.int64)
+    u = np.frombuffer(raw[24:], dtype=np.float64).reshape(nelem, nvars, (n_poly + 1) ** 3)
+    return np.array(u), int(n_poly)
+
+
+def _have_exe():
+    return os.path.exists(EXE)
+
+
+@pytest.mark.skipif(not _have_exe(), reason="Fortran driver not built")
+def test_driver_reports_errors_without_device(tmp_path):
+    env = dict(os.environ)
+    env["HIP_VISIBLE_DEVICES"] = "-1"  # no device visible: creation must fail cleanly
+    r = subprocess.run([EXE, "vortex", "2", "3", "1", str(tmp_path / "x")], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode != 0
+    assert "[ERROR] nemo_gpu_create" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not _have_exe(), reason="Fortran driver not built")
+@pytest.mark.parametrize("setup", ["vortex", "kh"])
+def test_fortran_driver_matches_oracle(tmp_path, setup):
+    n, N, steps = 8, 3, 20
+    prefix = str(tmp_path / setup)
+    print(f"fortran driver {setup} n={n} N={N} steps={steps}", flush=True)
+    r = subprocess.run([EXE, setup, str(n), str(N), str(steps), prefix], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    u0, n_poly = _read_state(prefix + ".init.bin")
+    u1, _ = _read_state(prefix + ".final.bin")
+    dts = np.loadtxt(prefix + ".dt.txt")
+    assert n_poly == N and len(dts) == steps
+    m = meshmod.uniform_periodic(n)
+    if setup == "vortex":
+        cfg = O.make_cfg(N=N, volume=O.VOL_WEAK, surface=O.SURF_RUSANOV, timedisc=O.TD_SSPRK54)
+    else:
+        cfg = O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+                         repair=1)
+    uref = np.ascontiguousarray(u0)
+    rc, href = O.run(cfg, m.nbr, m.delta, uref, steps)
+    assert rc == 0
+    assert np.abs(dts - href).max() <= 1e-13 * href.max()
+    err = np.abs(u1 - uref).max() / np.abs(uref).max()
+    print(f"  rel Linf after {steps} steps {err:.3e}")
+    assert err <= 1e-10

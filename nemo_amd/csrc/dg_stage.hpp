@@ -85,6 +85,29 @@ __device__ __forceinline__ constexpr int nidx(int l, int a, int b) {
     return D == 0 ? l + M * a + M * M * b : (D == 1 ? a + M * l + M * M * b : a + M * b + M * M * l);
 }
 
+// LDS slot of node n = i + M*j + M*M*k inside an element.  For M = 4 the
+// slot is 16k + 4(i^j) + (i^k): a bijection under which the 64 lanes of any
+// direction pass (one node per line, 16 lines x 4 elements) read 16 distinct
+// 8-byte bank pairs, 4 lanes each -- the minimum of 4 LDS passes for a
+// 64-lane b64 access.  Without it the x- and y-passes hit only 4 bank pairs
+// (16-way conflicts).  Other M keep the natural order.
+template <int M>
+__device__ __forceinline__ constexpr int swz(int n) {
+    if constexpr (M == 4) {
+        const int i = n & 3, j = (n >> 2) & 3, k = n >> 4;
+        return 16 * k + 4 * (i ^ j) + (i ^ k);
+    } else {
+        return n;
+    }
+}
+
+// LDS slot of block-local node index idx = element*M^3 + n.
+template <int M>
+__device__ __forceinline__ constexpr int slot(int idx) {
+    constexpr int M3 = M * M * M;
+    return (idx / M3) * M3 + swz<M>(idx % M3);
+}
+
 template <int SURF, int D, typename Real>
 __device__ __forceinline__ void riemann(const Gas<Real>& g, const NodeState<Real>& uL, const NodeState<Real>& uR,
                                         Real* f) {
@@ -136,7 +159,7 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     NodeState<Real> S[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + nidx<D, M>(i, ta, tb);
+        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
 #pragma unroll
         for (int h = 0; h < 5; ++h) S[i].u[h] = Real(s_u[h][n]);
         make_state<D, BETA>(g, S[i]);
@@ -267,7 +290,7 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
 
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + nidx<D, M>(i, ta, tb);
+        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
 #pragma unroll
         for (int h = 0; h < 5; ++h) {
             if constexpr (D == 0)
@@ -293,12 +316,14 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
     Real* t2 = s_acc[1];
     for (int idx = tid; idx < NN; idx += NT) {
         if constexpr (sizeof(Real) == 4) {
-            q[idx] = Real(s_u[DENS][idx] * s_p64[idx]);
+            const int sl = slot<M>(idx);
+            q[sl] = Real(s_u[DENS][sl] * s_p64[sl]);
         } else {
+            const int sl = slot<M>(idx);
             double uu[5];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
-            q[idx] = uu[DENS] * pressure(a.gas64, uu);
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][sl];
+            q[sl] = uu[DENS] * pressure(a.gas64, uu);
         }
     }
     __syncthreads();
@@ -308,33 +333,33 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
     {  // x: t1(a,j,k) = sum_i Vinv(a,i) q(i,j,k)
         Real v[M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) v[i] = q[base + i + M * ta + M2 * tb];
+        for (int i = 0; i < M; ++i) v[i] = q[base + swz<M>(i + M * ta + M2 * tb)];
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
 #pragma unroll
             for (int i = 0; i < M; ++i) s += a.op.Vinv[r][i] * v[i];
-            t1[base + r + M * ta + M2 * tb] = s;
+            t1[base + swz<M>(r + M * ta + M2 * tb)] = s;
         }
     }
     __syncthreads();
     {  // y: t2(i,b,k) = sum_j Vinv(b,j) t1(i,j,k)
         Real v[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) v[j] = t1[base + ta + M * j + M2 * tb];
+        for (int j = 0; j < M; ++j) v[j] = t1[base + swz<M>(ta + M * j + M2 * tb)];
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
 #pragma unroll
             for (int j = 0; j < M; ++j) s += a.op.Vinv[r][j] * v[j];
-            t2[base + ta + M * r + M2 * tb] = s;
+            t2[base + swz<M>(ta + M * r + M2 * tb)] = s;
         }
     }
     __syncthreads();
     {  // z: modal(a,b,c) = sum_k Vinv(c,k) t2(a,b,k); energies
         Real v[M];
 #pragma unroll
-        for (int k = 0; k < M; ++k) v[k] = t2[base + ta + M * tb + M2 * k];
+        for (int k = 0; k < M; ++k) v[k] = t2[base + swz<M>(ta + M * tb + M2 * k)];
         Real tot = Real(0), c1 = Real(0), c2 = Real(0);
 #pragma unroll
         for (int r = 0; r < M; ++r) {
@@ -386,7 +411,7 @@ template <int M, typename Real>
 __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* su, int base) {
     auto node_ok = [&](int n) {
         Real uu[5];
-        for (int h = 0; h < 5; ++h) uu[h] = su[h][base + n];
+        for (int h = 0; h < 5; ++h) uu[h] = su[h][base + swz<M>(n)];
         return isvalid(g, uu);
     };
     auto all_ok = [&]() {
@@ -405,7 +430,7 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
                 for (int i = 0; i < M; ++i) {
                     const Real ww = w[i] * w[j] * w[k];
                     wsum += ww;
-                    for (int h = 0; h < 5; ++h) means[h] += ww * su[h][base + i + M * j + M * M * k];
+                    for (int h = 0; h < 5; ++h) means[h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
                 }
         for (int h = 0; h < 5; ++h) means[h] = means[h] / wsum;
         // squeeze all nodes towards the mean in repair_decr steps
@@ -417,7 +442,7 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
             const Real f = squeeze / scale_prev;  // rescale current deviations
             for (int n = 0; n < M * M * M; ++n)
                 for (int h = 0; h < 5; ++h) {
-                    Real* x = &su[h][base + n];
+                    Real* x = &su[h][base + swz<M>(n)];
                     *x = means[h] + (squeeze == Real(0) ? Real(0) : f) * (*x - means[h]);
                 }
             scale_prev = squeeze;
@@ -446,7 +471,7 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
                                             const Real ww = w[i] * w[j] * w[k];
                                             tw[t] += ww;
                                             for (int h = 0; h < 5; ++h)
-                                                tiles[t][h] += ww * su[h][base + i + M * j + M * M * k];
+                                                tiles[t][h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
                                         }
                                 for (int h = 0; h < 5; ++h) tiles[t][h] = tiles[t][h] / tw[t];
                             }
@@ -477,7 +502,7 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
                         for (int j = 2 * p * nn; j < (2 * p + 2) * nn; ++j)
                             for (int i = 2 * q * nn; i < (2 * q + 2) * nn; ++i)
                                 for (int h = 0; h < 5; ++h) {
-                                    Real* x = &su[h][base + i + M * j + M * M * k];
+                                    Real* x = &su[h][base + swz<M>(i + M * j + M * M * k)];
                                     *x = means[h] + squeeze * (*x - means[h]);
                                 }
                 }
@@ -503,7 +528,7 @@ __device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, vm = 0.0;
     if (e0 + el < nelem) {
         for (int i = 0; i < M; ++i) {
-            const int n = el * M3 + i + M * line;
+            const int n = el * M3 + swz<M>(i + M * line);
             double uu[5];
 #pragma unroll
             for (int h = 0; h < 5; ++h) uu[h] = s_u[h][n];
@@ -582,7 +607,8 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
 #pragma unroll
     for (int h = 0; h < 5; ++h)
         for (int idx = tid; idx < NN; idx += NT)
-            s_u[h][idx] = idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
+            s_u[h][slot<M>(idx)] =
+                idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
     if (tid < BE) {
         s_alpha[tid] = Real(0);
         s_bad[tid] = 0;
@@ -590,10 +616,11 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     __syncthreads();
     if constexpr (F32) {
         for (int idx = tid; idx < NN; idx += NT) {
+            const int sl = slot<M>(idx);
             double uu[5];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
-            s_p64[idx] = pressure(a.gas64, uu);
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][sl];
+            s_p64[sl] = pressure(a.gas64, uu);
         }
         __syncthreads();
     }
@@ -612,7 +639,8 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     if (a.rhs_out) {
 #pragma unroll
         for (int h = 0; h < 5; ++h)
-            for (int idx = tid; idx < nvalid_nodes; idx += NT) a.rhs_out[h * EM3 + e0 * M3 + idx] = double(s_acc[h][idx]);
+            for (int idx = tid; idx < nvalid_nodes; idx += NT)
+                a.rhs_out[h * EM3 + e0 * M3 + idx] = double(s_acc[h][slot<M>(idx)]);
         return;
     }
 
@@ -624,8 +652,9 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     for (int h = 0; h < 5; ++h)
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             const long long gi = h * EM3 + e0 * M3 + idx;
-            const double rhs = double(s_acc[h][idx]);
-            const double u = s_u[h][idx];
+            const int sl = slot<M>(idx);
+            const double rhs = double(s_acc[h][sl]);
+            const double u = s_u[h][sl];
             double st;
             if (a.timedisc == TD_LSRK54) {
                 const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * a.r0[gi] + rhs;
@@ -644,7 +673,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
                 default: st = k[17] * a.r0[gi] + k[18] * u + k[19] * dt * rhs + a.r1[gi]; break;
                 }
             }
-            s_u[h][idx] = st;
+            s_u[h][sl] = st;
         }
     __syncthreads();
 
@@ -653,7 +682,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             double uu[5];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][idx];
+            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][slot<M>(idx)];
             if (!isvalid(a.gas64, uu)) s_bad[idx / M3] = 1;
         }
         __syncthreads();
@@ -669,7 +698,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     for (int h = 0; h < 5; ++h)
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             const long long gi = h * EM3 + e0 * M3 + idx;
-            const double st = s_u[h][idx];
+            const double st = s_u[h][slot<M>(idx)];
             a.u_out[gi] = st;
             if (a.timedisc == TD_SSPRK54) {
                 if (s == 2) a.r1[gi] = a.rk.ssp[20] * st;
@@ -699,7 +728,7 @@ __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restric
 #pragma unroll
     for (int h = 0; h < 5; ++h)
         for (int idx = tid; idx < NN; idx += NT)
-            s_u[h][idx] = idx < nvalid ? u[h * EM3 + e0 * M3 + idx] : (h == 0 || h == 4 ? 1.0 : 0.0);
+            s_u[h][slot<M>(idx)] = idx < nvalid ? u[h * EM3 + e0 * M3 + idx] : (h == 0 || h == 4 ? 1.0 : 0.0);
     __syncthreads();
     block_cfl<M, BE, NT>(g, delta, e0, nelem, s_u, &s_w[0][0], part + 4 * blk);
 }

@@ -464,6 +464,426 @@ void orc_two_point(const orc_cfg *c, int kind, int dir, const double *u, const d
     rot_out(dir, rf, f);
 }
 
+
+/* ======================================================================== */
+/* Ideal GLM-MHD (9 variables): equations/mhd/equations_mod.f90             */
+/* ======================================================================== */
+
+enum { MAGX = 5, MAGY = 6, MAGZ = 7, GLMP = 8 };
+
+/* rotation of momentum and magnetic field together: equations_mod.f90:140-218 */
+static void rot_in9(int dir, const double *u, double *r) {
+    static const int p[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    r[DENS] = u[DENS];
+    r[ENER] = u[ENER];
+    r[GLMP] = u[GLMP];
+    for (int k = 0; k < 3; ++k) {
+        r[MOMX + k] = u[MOMX + p[dir][k]];
+        r[MAGX + k] = u[MAGX + p[dir][k]];
+    }
+}
+static void rot_out9(int dir, const double *r, double *f) {
+    static const int p[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    f[DENS] = r[DENS];
+    f[ENER] = r[ENER];
+    f[GLMP] = r[GLMP];
+    for (int k = 0; k < 3; ++k) {
+        f[MOMX + p[dir][k]] = r[MOMX + k];
+        f[MAGX + p[dir][k]] = r[MAGX + k];
+    }
+}
+
+/* pressure_elemental: equations/mhd/equations_mod.f90:224-234 */
+static double pres9(const orc_cfg *c, const double *u) {
+    return (c->gamma - 1.0) * (u[ENER] - 0.5 / u[DENS] * (u[MOMX] * u[MOMX] + u[MOMY] * u[MOMY] + u[MOMZ] * u[MOMZ]) -
+                               0.5 / c->mu0 * (u[MAGX] * u[MAGX] + u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ] + u[GLMP] * u[GLMP]));
+}
+
+/* xflux with GLM terms: equations/mhd/equations_mod.f90:64-138 */
+static void xflux9(const orc_cfg *c, const double *u, double *f) {
+    const double kappa = c->gamma, mu0 = c->mu0;
+    const double smu_0 = 1.0 / mu0, s2mu_0 = 0.5 / mu0;
+    const double kappaM1 = kappa - 1.0, kappaM2 = kappa - 2.0;
+    const double rho = u[0], rhov1 = u[1], rhov2 = u[2], rhov3 = u[3];
+    const double Etotal = u[4] - 0.5 * smu_0 * (u[8] * u[8]);
+    const double b1 = u[5], b2 = u[6], b3 = u[7];
+    const double srho = 1.0 / rho;
+    const double v1 = rhov1 * srho, v2 = rhov2 * srho, v3 = rhov3 * srho;
+    const double v_2 = v1 * v1 + v2 * v2 + v3 * v3;
+    const double bb2 = (b1 * b1 + b2 * b2 + b3 * b3);
+    const double vb = (b1 * v1 + b2 * v2 + b3 * v3);
+    const double pt = kappaM1 * (Etotal - 0.5 * rho * (v_2)) - kappaM2 * s2mu_0 * bb2;
+    const double Ep = (Etotal + pt);
+    f[0] = rhov1;
+    f[1] = rhov1 * v1 + pt - smu_0 * b1 * b1;
+    f[2] = rhov1 * v2 - smu_0 * b1 * b2;
+    f[3] = rhov1 * v3 - smu_0 * b1 * b3;
+    f[4] = Ep * v1 - smu_0 * b1 * vb;
+    f[5] = 0.0;
+    f[6] = v1 * b2 - b1 * v2;
+    f[7] = v1 * b3 - b1 * v3;
+    f[4] = f[4] + smu_0 * c->glm_ch * b1 * u[8];
+    f[5] = f[5] + c->glm_ch * u[8];
+    f[8] = c->glm_ch * b1;
+}
+
+/* Fastest_Signal_Speed: equations/mhd/equations_mod.f90:693-714 */
+static double fastspeed9(const orc_cfg *c, const double *u) {
+    const double kappa = c->gamma, mu0 = c->mu0;
+    const double pres = pres9(c, u);
+    const double cs2 = kappa * pres / u[DENS];
+    const double ca2 = u[MAGX] * u[MAGX] / u[DENS] / mu0;
+    const double va2 = (u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ]) / u[DENS] / mu0 + ca2;
+    const double astar = sqrt((cs2 + va2) * (cs2 + va2) - 4.0 * cs2 * ca2);
+    return sqrt(0.5 * (cs2 + va2 + astar));
+}
+
+/* xlambdaMax_elemental: equations/mhd/equations_mod.f90:522-531 */
+static double xlmax9(const orc_cfg *c, const double *u) { return fabs(u[MOMX] / u[DENS]) + fastspeed9(c, u); }
+
+/* Rusanov: equations/mhd/riemann/rusanov/riemann_mod.f90:7-24 */
+static void rusanov9_x(const orc_cfg *c, const double *uL, const double *uR, double *sf) {
+    double FL[9], FR[9];
+    xflux9(c, uL, FL);
+    xflux9(c, uR, FR);
+    const double a = xlmax9(c, uL), b = xlmax9(c, uR);
+    const double lmax = a > b ? a : b;
+    for (int h = 0; h < 9; ++h) sf[h] = 0.5 * (FL[h] + FR[h] + lmax * (uL[h] - uR[h]));
+}
+
+/* Powell + GLM non-conservative two-point term:
+ * equations/mhd/non_conservative_flux_mod.f90:10-45 */
+static void nc9_x(const orc_cfg *c, const double *UI, const double *UL, const double *UR, double *fl) {
+    const double mu0 = c->mu0;
+    const double Bavg = 0.5 * (UL[MAGX] + UR[MAGX]);
+    const double Gavg = 0.5 * (UL[GLMP] + UR[GLMP]);
+    const double v[3] = {UI[MOMX] / UI[DENS], UI[MOMY] / UI[DENS], UI[MOMZ] / UI[DENS]};
+    for (int h = 0; h < 9; ++h) fl[h] = 0.0;
+    for (int k = 0; k < 3; ++k) fl[MOMX + k] = Bavg * UI[MAGX + k] / mu0;
+    fl[ENER] = Bavg * (UI[MAGX] * v[0] + UI[MAGY] * v[1] + UI[MAGZ] * v[2]) / mu0;
+    for (int k = 0; k < 3; ++k) fl[MAGX + k] = Bavg * v[k];
+    fl[ENER] = fl[ENER] + Gavg * v[0] * (UI[GLMP] / mu0);
+    fl[GLMP] = fl[GLMP] + Gavg * v[0] * 1.0;
+}
+
+/* LN_MEAN: equations/mhd/two_point/ec/two_point_flux_mod.f90:203-219 */
+static double ln_mean9(double aL, double aR) {
+    const double Xi = aR / aL;
+    const double u = (Xi * (Xi - 2.0) + 1.0) / (Xi * (Xi + 2.0) + 1.0);
+    if (u < 1.0e-4) return (aL + aR) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + u * 15.0)));
+    return (aR - aL) / log(Xi);
+}
+
+/* Entropy-conserving GLM-MHD flux (Derigs et al.):
+ * equations/mhd/two_point/ec/two_point_flux_mod.f90:13-86.  The reference's
+ * y/z versions do not compile (they call xflux with one argument); y/z
+ * here rotate like every other directional function of the reference. */
+static void ec9_x(const orc_cfg *c, const double *UL, const double *UR, double *Fs) {
+    const double kappa = c->gamma, mu0 = c->mu0;
+    const double kappaM1 = kappa - 1.0, skappaM1 = 1.0 / (kappa - 1.0), smu_0 = 1.0 / mu0;
+    const double rho_L = UL[0], rho_R = UR[0];
+    const double E_L = UL[4] - 0.5 * smu_0 * (UL[8] * UL[8]), E_R = UR[4] - 0.5 * smu_0 * (UR[8] * UR[8]);
+    const double psi_L = UL[8], psi_R = UR[8];
+    const double *B_L = UL + 5, *B_R = UR + 5;
+    double v_L[3], v_R[3];
+    for (int k = 0; k < 3; ++k) {
+        v_L[k] = UL[1 + k] / rho_L;
+        v_R[k] = UR[1 + k] / rho_R;
+    }
+    const double v2_L = v_L[0] * v_L[0] + v_L[1] * v_L[1] + v_L[2] * v_L[2];
+    const double v2_R = v_R[0] * v_R[0] + v_R[1] * v_R[1] + v_R[2] * v_R[2];
+    const double B2_L = B_L[0] * B_L[0] + B_L[1] * B_L[1] + B_L[2] * B_L[2];
+    const double B2_R = B_R[0] * B_R[0] + B_R[1] * B_R[1] + B_R[2] * B_R[2];
+    const double p_L = kappaM1 * (E_L - 0.5 * (rho_L * v2_L + smu_0 * B2_L));
+    const double p_R = kappaM1 * (E_R - 0.5 * (rho_R * v2_R + smu_0 * B2_R));
+    const double beta_L = 0.5 * rho_L / p_L;
+    const double beta_R = 0.5 * rho_R / p_R;
+    const double rhoLN = ln_mean9(rho_L, rho_R);
+    const double betaLN = ln_mean9(beta_L, beta_R);
+    double vAvg[3], BAvg[3];
+    for (int k = 0; k < 3; ++k) {
+        vAvg[k] = 0.5 * (v_L[k] + v_R[k]);
+        BAvg[k] = 0.5 * (B_L[k] + B_R[k]);
+    }
+    const double v1_B2Avg = 0.5 * (v_L[0] * B2_L + v_R[0] * B2_R);
+    const double vB_Avg = 0.5 * ((v_L[0] * B_L[0] + v_L[1] * B_L[1] + v_L[2] * B_L[2]) +
+                                 (v_R[0] * B_R[0] + v_R[1] * B_R[1] + v_R[2] * B_R[2]));
+    const double pTilde = 0.5 * ((rho_L + rho_R) / (beta_L + beta_R) + smu_0 * 0.5 * (B2_L + B2_R));
+    const double psiAvg = 0.5 * (psi_L + psi_R);
+    Fs[0] = rhoLN * vAvg[0];
+    Fs[1] = Fs[0] * vAvg[0] - smu_0 * BAvg[0] * BAvg[0] + pTilde;
+    Fs[2] = Fs[0] * vAvg[1] - smu_0 * BAvg[0] * BAvg[1];
+    Fs[3] = Fs[0] * vAvg[2] - smu_0 * BAvg[0] * BAvg[2];
+    Fs[6] = vAvg[0] * BAvg[1] - BAvg[0] * vAvg[1];
+    Fs[7] = vAvg[0] * BAvg[2] - BAvg[0] * vAvg[2];
+    Fs[5] = c->glm_ch * psiAvg;
+    Fs[8] = c->glm_ch * BAvg[0];
+    Fs[4] = Fs[0] * 0.5 * (skappaM1 / betaLN - 0.5 * (v2_L + v2_R)) +
+            (vAvg[0] * Fs[1] + vAvg[1] * Fs[2] + vAvg[2] * Fs[3]) +
+            smu_0 * ((BAvg[0] * Fs[5] + BAvg[1] * Fs[6] + BAvg[2] * Fs[7]) - 0.5 * v1_B2Avg + BAvg[0] * vB_Avg +
+                     Fs[8] * psiAvg - c->glm_ch * 0.5 * (psi_L * B_L[0] + psi_R * B_R[0]));
+}
+
+/* cons2prim / prim2cons: equations/mhd/equations_mod.f90:276-310 */
+static void cons2prim9(const orc_cfg *c, const double *u, double *p) {
+    p[0] = u[0];
+    p[1] = u[1] / u[0];
+    p[2] = u[2] / u[0];
+    p[3] = u[3] / u[0];
+    p[4] = pres9(c, u);
+    p[5] = u[5]; p[6] = u[6]; p[7] = u[7]; p[8] = u[8];
+}
+static void prim2cons9(const orc_cfg *c, const double *p, double *u) {
+    u[0] = p[0];
+    u[1] = p[0] * p[1];
+    u[2] = p[0] * p[2];
+    u[3] = p[0] * p[3];
+    u[4] = p[4] / (c->gamma - 1.0) + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) +
+           0.5 / c->mu0 * ((p[5] * p[5] + p[6] * p[6] + p[7] * p[7]) + p[8] * p[8]);
+    u[5] = p[5]; u[6] = p[6]; u[7] = p[7]; u[8] = p[8];
+}
+
+/* HLLD: equations/mhd/riemann/hlld/riemann_mod.f90:16-248 (Miyoshi & Kusano,
+ * JCP 208, 2005; mu0 = 1 as in the reference).  The reference file carries an
+ * "# Error" guard (it does not compile and does not handle GLM).  Restated
+ * with two documented changes:
+ *  (1) the fast speeds come from the conservative states (the reference
+ *      passes primitive arrays to Fastest_Signal_Speed, which expects
+ *      conservative ones);
+ *  (2) GLM: the (B_n, psi) subsystem is solved exactly first (Dedner et al.
+ *      JCP 175, 2002; Mignone & Tzeferacos JCP 229, 2010):
+ *        B_n* = (B_nL + B_nR)/2 + (psi_L - psi_R)/2,
+ *        psi* = (psi_L + psi_R)/2 + (B_nL - B_nR)/2,
+ *      fluxes F(B_n) = c_h psi*, F(psi) = c_h B_n*, energy + c_h B_n* psi* / mu0;
+ *      the seven-wave HLLD then runs with B_n = B_n* on both sides (the
+ *      reference's B_n averaging, :71-73) on the energy without psi^2/2. */
+static void hlld9_x(const orc_cfg *c, const double *UL_in, const double *UR_in, double *Flux) {
+    const double hlld_eps = 1.0e-8, hlld_small_eps = 1.0e-12;
+    const double ch = c->glm_ch, mu0 = c->mu0;
+    const double Bstar = 0.5 * (UL_in[MAGX] + UR_in[MAGX]) + 0.5 * (UL_in[GLMP] - UR_in[GLMP]);
+    const double psistar = 0.5 * (UL_in[GLMP] + UR_in[GLMP]) + 0.5 * (UL_in[MAGX] - UR_in[MAGX]);
+    orc_cfg c0 = *c;
+    c0.glm_ch = 0.0;
+    double UL[9], UR[9], PrimL[9], PrimR[9], ConsL[9], ConsR[9], FluxL[9], FluxR[9];
+    for (int h = 0; h < 9; ++h) { UL[h] = UL_in[h]; UR[h] = UR_in[h]; }
+    /* energy without the psi^2/2 part, psi removed from the MHD system */
+    UL[ENER] = UL[ENER] - 0.5 / mu0 * (UL[GLMP] * UL[GLMP]);
+    UR[ENER] = UR[ENER] - 0.5 / mu0 * (UR[GLMP] * UR[GLMP]);
+    UL[GLMP] = 0.0;
+    UR[GLMP] = 0.0;
+    cons2prim9(&c0, UL, PrimL);
+    cons2prim9(&c0, UR, PrimR);
+    const double Bn = Bstar;
+    PrimR[5] = Bn;
+    PrimL[5] = Bn;
+    prim2cons9(&c0, PrimL, ConsL);
+    prim2cons9(&c0, PrimR, ConsR);
+    const double cf_L = fastspeed9(&c0, ConsL);
+    const double cf_R = fastspeed9(&c0, ConsR);
+    const double FastestWave = cf_L > cf_R ? cf_L : cf_R;
+    const double SL = (PrimL[1] < PrimR[1] ? PrimL[1] : PrimR[1]) - FastestWave;
+    const double SR = (PrimL[1] > PrimR[1] ? PrimL[1] : PrimR[1]) + FastestWave;
+    if (SL > 0.0) {
+        xflux9(&c0, ConsL, Flux);
+    } else if (SR < 0.0) {
+        xflux9(&c0, ConsR, Flux);
+    } else {
+        xflux9(&c0, ConsL, FluxL);
+        xflux9(&c0, ConsR, FluxR);
+        const double rhoL = PrimL[0], rhoR = PrimR[0], unL = PrimL[1], unR = PrimR[1];
+        const double pL = PrimL[4], pR = PrimR[4], eL = ConsL[4], eR = ConsR[4];
+        const double *uL = PrimL + 1, *uR = PrimR + 1, *BL = PrimL + 5, *BR = PrimR + 5;
+        double utL[3] = {0.0, uL[1], uL[2]}, utR[3] = {0.0, uR[1], uR[2]};
+        double BtL[3] = {0.0, BL[1], BL[2]}, BtR[3] = {0.0, BR[1], BR[2]};
+        const double pmagL = 0.5 * (BL[0] * BL[0] + BL[1] * BL[1] + BL[2] * BL[2]), ptotL = pL + pmagL;
+        const double pmagR = 0.5 * (BR[0] * BR[0] + BR[1] * BR[1] + BR[2] * BR[2]), ptotR = pR + pmagR;
+        const double SM = (rhoR * unR * (SR - unR) - rhoL * unL * (SL - unL) - ptotR + ptotL) /
+                          (rhoR * (SR - unR) - rhoL * (SL - unL));
+        const double sSL_SM = 1.0 / (SL - SM), sSR_SM = 1.0 / (SR - SM);
+        const double rhoLS = rhoL * (SL - unL) * sSL_SM;
+        const double rhoRS = rhoR * (SR - unR) * sSR_SM;
+        const double sqrtRhoLS = sqrt(rhoLS), sqrtRhoRS = sqrt(rhoRS);
+        const double SLS = SM - fabs(Bn) / sqrtRhoLS;
+        const double SRS = SM + fabs(Bn) / sqrtRhoRS;
+        const double ptotS = ptotL + rhoL * (SL - unL) * (SM - unL);
+        double utLS[3], BtLS[3], utRS[3], BtRS[3], uLS[3], BLS[3], uRS[3], BRS[3];
+        double temp = rhoL * (SL - unL) * (SL - SM) - Bn * Bn;
+        if (fabs(temp) < hlld_eps) {
+            for (int k = 0; k < 3; ++k) { utLS[k] = utL[k]; BtLS[k] = BtL[k]; }
+        } else {
+            const double stemp = 1. / temp;
+            for (int k = 0; k < 3; ++k) {
+                utLS[k] = utL[k] - (Bn * (SM - unL) * stemp) * BtL[k];
+                BtLS[k] = ((rhoL * (SL - unL) * (SL - unL) - Bn * Bn) * stemp) * BtL[k];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            BLS[k] = (k == 0 ? Bn : 0.0) + BtLS[k];
+            uLS[k] = (k == 0 ? SM : 0.0) + utLS[k];
+        }
+        double uBS = uLS[0] * BLS[0] + uLS[1] * BLS[1] + uLS[2] * BLS[2];
+        double uB = uL[0] * BL[0] + uL[1] * BL[1] + uL[2] * BL[2];
+        const double eLS = (eL * (SL - unL) + ptotS * SM - ptotL * unL + Bn * (uB - uBS)) * sSL_SM;
+        double U_LS[9], U_RS[9], U_LSS[9], U_RSS[9];
+        U_LS[0] = rhoLS;
+        for (int k = 0; k < 3; ++k) U_LS[1 + k] = rhoLS * uLS[k];
+        U_LS[4] = eLS;
+        for (int k = 0; k < 3; ++k) U_LS[5 + k] = BLS[k];
+        U_LS[8] = ConsL[8];
+        temp = rhoR * (SR - unR) * (SR - SM) - Bn * Bn;
+        if (fabs(temp) < hlld_eps) {
+            for (int k = 0; k < 3; ++k) { utRS[k] = utR[k]; BtRS[k] = BtR[k]; }
+        } else {
+            const double stemp = 1. / temp;
+            for (int k = 0; k < 3; ++k) {
+                utRS[k] = utR[k] - (Bn * (SM - unR) * stemp) * BtR[k];
+                BtRS[k] = ((rhoR * (SR - unR) * (SR - unR) - Bn * Bn) * stemp) * BtR[k];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            BRS[k] = (k == 0 ? Bn : 0.0) + BtRS[k];
+            uRS[k] = (k == 0 ? SM : 0.0) + utRS[k];
+        }
+        uBS = uRS[0] * BRS[0] + uRS[1] * BRS[1] + uRS[2] * BRS[2];
+        uB = uR[0] * BR[0] + uR[1] * BR[1] + uR[2] * BR[2];
+        const double eRS = (eR * (SR - unR) - ptotR * unR + ptotS * SM + Bn * (uB - uBS)) * sSR_SM;
+        U_RS[0] = rhoRS;
+        for (int k = 0; k < 3; ++k) U_RS[1 + k] = rhoRS * uRS[k];
+        U_RS[4] = eRS;
+        for (int k = 0; k < 3; ++k) U_RS[5 + k] = BRS[k];
+        U_RS[8] = ConsR[8];
+        if (fabs(Bn) < hlld_small_eps) {
+            for (int h = 0; h < 9; ++h) { U_LSS[h] = U_LS[h]; U_RSS[h] = U_RS[h]; }
+        } else {
+            const double signBn = Bn < 0.0 ? -1.0 : 1.0;
+            const double stemp = 1. / (sqrtRhoLS + sqrtRhoRS);
+            double utSS[3], BtSS[3], uSS[3], BSS[3];
+            for (int k = 0; k < 3; ++k) {
+                utSS[k] = (sqrtRhoLS * utLS[k] + sqrtRhoRS * utRS[k] + signBn * (BtRS[k] - BtLS[k])) * stemp;
+                BtSS[k] = (sqrtRhoLS * BtRS[k] + sqrtRhoRS * BtLS[k] + signBn * (utRS[k] - utLS[k]) * sqrtRhoLS * sqrtRhoRS) * stemp;
+                uSS[k] = (k == 0 ? SM : 0.0) + utSS[k];
+                BSS[k] = (k == 0 ? Bn : 0.0) + BtSS[k];
+            }
+            uBS = utSS[0] * BtSS[0] + utSS[1] * BtSS[1] + utSS[2] * BtSS[2];
+            uB = utLS[0] * BtLS[0] + utLS[1] * BtLS[1] + utLS[2] * BLS[2];
+            const double eLSS = eLS - signBn * sqrtRhoLS * (uB - uBS);
+            uB = uRS[0] * BtRS[0] + utRS[1] * BtRS[1] + utRS[2] * BtRS[2];
+            const double eRSS = eRS + signBn * sqrtRhoRS * (uB - uBS);
+            U_LSS[0] = rhoLS; U_RSS[0] = rhoRS;
+            for (int k = 0; k < 3; ++k) { U_LSS[1 + k] = rhoLS * uSS[k]; U_RSS[1 + k] = rhoRS * uSS[k]; }
+            U_LSS[4] = eLSS; U_RSS[4] = eRSS;
+            for (int k = 0; k < 3; ++k) { U_LSS[5 + k] = BSS[k]; U_RSS[5 + k] = BSS[k]; }
+            U_LSS[8] = ConsL[8]; U_RSS[8] = ConsR[8];
+        }
+        for (int h = 0; h < 9; ++h) {
+            if (0.0 <= SLS)
+                Flux[h] = FluxL[h] + SL * (U_LS[h] - ConsL[h]);
+            else if (0.0 <= SM)
+                Flux[h] = FluxL[h] + SLS * U_LSS[h] - (SLS - SL) * U_LS[h] - SL * ConsL[h];
+            else if (0.0 <= SRS)
+                Flux[h] = FluxR[h] + SRS * U_RSS[h] - (SRS - SR) * U_RS[h] - SR * ConsR[h];
+            else
+                Flux[h] = FluxR[h] + SR * (U_RS[h] - ConsR[h]);
+        }
+    }
+    /* GLM subsystem */
+    Flux[MAGX] = ch * psistar;
+    Flux[GLMP] = ch * Bstar;
+    Flux[ENER] = Flux[ENER] + ch * Bstar * psistar / mu0;
+}
+
+/* ---- dispatch over the equations ----------------------------------------- */
+
+static void phys_flux(const orc_cfg *c, int dir, const double *u, double *f) {
+    if (c->eq == ORC_EQ_MHD) {
+        double ru[9], rf[9];
+        rot_in9(dir, u, ru);
+        xflux9(c, ru, rf);
+        rot_out9(dir, rf, f);
+    } else {
+        orc_flux(c, dir, u, f);
+    }
+}
+
+static void phys_riemann(const orc_cfg *c, int kind, int dir, const double *uL, const double *uR, double *f) {
+    if (c->eq == ORC_EQ_MHD) {
+        double ruL[9], ruR[9], rf[9];
+        rot_in9(dir, uL, ruL);
+        rot_in9(dir, uR, ruR);
+        if (kind == ORC_SURF_HLLD)
+            hlld9_x(c, ruL, ruR, rf);
+        else if (kind == ORC_SURF_EC_TEST)
+            ec9_x(c, ruL, ruR, rf);
+        else
+            rusanov9_x(c, ruL, ruR, rf);
+        rot_out9(dir, rf, f);
+    } else {
+        orc_riemann(c, kind, dir, uL, uR, f);
+    }
+}
+
+static void phys_two_point(const orc_cfg *c, int kind, int dir, const double *u, const double *v, double *f) {
+    if (c->eq == ORC_EQ_MHD) {
+        double ru[9], rv[9], rf[9];
+        rot_in9(dir, u, ru);
+        rot_in9(dir, v, rv);
+        if (kind == ORC_VOL_EC) {
+            ec9_x(c, ru, rv, rf);
+        } else {
+            double fu[9], fv[9];
+            xflux9(c, ru, fu);
+            xflux9(c, rv, fv);
+            for (int h = 0; h < 9; ++h) rf[h] = 0.5 * (fu[h] + fv[h]);
+        }
+        rot_out9(dir, rf, f);
+    } else {
+        orc_two_point(c, kind, dir, u, v, f);
+    }
+}
+
+/* non-conservative term: zero for Euler */
+static void phys_nc(const orc_cfg *c, int dir, const double *uI, const double *uL, const double *uR, double *f) {
+    if (c->eq == ORC_EQ_MHD) {
+        double ri[9], rl[9], rr[9], rf[9];
+        rot_in9(dir, uI, ri);
+        rot_in9(dir, uL, rl);
+        rot_in9(dir, uR, rr);
+        nc9_x(c, ri, rl, rr, rf);
+        rot_out9(dir, rf, f);
+    } else {
+        for (int h = 0; h < 5; ++h) f[h] = 0.0;
+    }
+}
+
+static double phys_pressure(const orc_cfg *c, const double *u) {
+    return c->eq == ORC_EQ_MHD ? pres9(c, u) : pres5(c->gamma, u);
+}
+
+static double phys_lambda_max(const orc_cfg *c, int dir, const double *u) {
+    if (c->eq == ORC_EQ_MHD) {
+        double ru[9];
+        rot_in9(dir, u, ru);
+        return xlmax9(c, ru);
+    }
+    return orc_lambda_max(c, dir, u);
+}
+
+/* exported MHD pointwise functions (for the tests) */
+double orc_mhd_pressure(const orc_cfg *c, const double *u) { return pres9(c, u); }
+double orc_mhd_fastspeed(const orc_cfg *c, const double *u) { return fastspeed9(c, u); }
+void orc_phys_flux(const orc_cfg *c, int dir, const double *u, double *f) { phys_flux(c, dir, u, f); }
+void orc_phys_riemann(const orc_cfg *c, int kind, int dir, const double *uL, const double *uR, double *f) {
+    phys_riemann(c, kind, dir, uL, uR, f);
+}
+void orc_phys_two_point(const orc_cfg *c, int kind, int dir, const double *u, const double *v, double *f) {
+    phys_two_point(c, kind, dir, u, v, f);
+}
+void orc_phys_nc(const orc_cfg *c, int dir, const double *uI, const double *uL, const double *uR, double *f) {
+    phys_nc(c, dir, uI, uL, uR, f);
+}
+double orc_phys_lambda_max(const orc_cfg *c, int dir, const double *u) { return phys_lambda_max(c, dir, u); }
+int orc_phys_isvalid(const orc_cfg *c, const double *u) { return (u[DENS] > 0.0) && (phys_pressure(c, u) > 0.0); }
+
 /* ======================================================================== */
 /* Element operators                                                        */
 /* ======================================================================== */
@@ -483,10 +903,11 @@ static inline int node_index(int M, int d, int l, int a, int b) {
 static double indicator_element(const orc_cfg *c, const orc_ops *op, const double *ue, double *energy_out) {
     int M = op->M, M3 = op->M3;
     double q[MAXM * MAXM * MAXM], t1[MAXM * MAXM * MAXM], t2[MAXM * MAXM * MAXM];
-    double uu[5];
+    double uu[MAXV];
+    const int V = orc_nvars(c);
     for (int n = 0; n < M3; ++n) {
-        for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
-        q[n] = uu[DENS] * pres5(c->gamma, uu);
+        for (int h = 0; h < V; ++h) uu[h] = ue[h * M3 + n];
+        q[n] = uu[DENS] * phys_pressure(c, uu);
     }
     /* modal = (Vinv x Vinv x Vinv) q, sum-factorised x, then y, then z */
     for (int k = 0; k < M; ++k)
@@ -558,11 +979,12 @@ void orc_indicator(const orc_cfg *c, int64_t nelem, const double *u, double *alp
  *  blended:    rhs = (1-alpha) rhs_DG + alpha rhs_FV. */
 static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const int32_t *nbr,
                         const double *delta, const double *u, double alpha, double *out) {
-    const int M = op->M, M3 = op->M3, V = 5;
+    const int M = op->M, M3 = op->M3, V = orc_nvars(c);
+    const int mhd = c->eq == ORC_EQ_MHD;
     const double *ue = u + e * V * M3;
-    double sfl[3][5][MAXM * MAXM * MAXM];
-    double vfl[3][5][MAXM * MAXM * MAXM];
-    double fvr[3][5][MAXM * MAXM * MAXM];
+    static __thread double sfl[3][MAXV][MAXM * MAXM * MAXM];
+    static __thread double vfl[3][MAXV][MAXM * MAXM * MAXM];
+    static __thread double fvr[3][MAXV][MAXM * MAXM * MAXM];
     int do_fv = (c->indicator != ORC_IND_NONE) && alpha > 0.0;
 
     for (int d = 0; d < 3; ++d) {
@@ -572,20 +994,33 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
         const double jac = 2.0 / dx;
         for (int b = 0; b < M; ++b)
             for (int a = 0; a < M; ++a) {
-                double ul[MAXM][5], unm[5], unp[5], fm[5], fp[5];
+                double ul[MAXM][MAXV], unm[MAXV], unp[MAXV], fm[MAXV], fp[MAXV];
                 for (int l = 0; l < M; ++l)
                     for (int h = 0; h < V; ++h) ul[l][h] = ue[h * M3 + node_index(M, d, l, a, b)];
                 for (int h = 0; h < V; ++h) {
                     unm[h] = u[em * V * M3 + h * M3 + node_index(M, d, M - 1, a, b)];
                     unp[h] = u[ep * V * M3 + h * M3 + node_index(M, d, 0, a, b)];
                 }
-                orc_riemann(c, c->surface, d, unm, ul[0], fm);
-                orc_riemann(c, c->surface, d, ul[M - 1], unp, fp);
+                phys_riemann(c, c->surface, d, unm, ul[0], fm);
+                phys_riemann(c, c->surface, d, ul[M - 1], unp, fp);
+                if (mhd) {
+                    /* Powell/GLM terms at the faces with the element's own
+                     * node as inner state and the face average (the
+                     * reference's faceavgs, kernel/dgsem/weakform/2d/
+                     * kernel_flux_mod.f90:156-192) */
+                    double nm[MAXV], np[MAXV];
+                    phys_nc(c, d, ul[0], unm, ul[0], nm);
+                    phys_nc(c, d, ul[M - 1], ul[M - 1], unp, np);
+                    for (int h = 0; h < V; ++h) {
+                        fm[h] = fm[h] + nm[h];
+                        fp[h] = fp[h] + np[h];
+                    }
+                }
 
                 if (c->volume == ORC_VOL_WEAK) {
-                    double fx[MAXM][5];
+                    double fx[MAXM][MAXV];
                     for (int l = 0; l < M; ++l) {
-                        orc_flux(c, d, ul[l], fx[l]);
+                        phys_flux(c, d, ul[l], fx[l]);
                         for (int h = 0; h < V; ++h) fx[l][h] = sdx * fx[l][h];
                     }
                     for (int i = 0; i < M; ++i) {
@@ -598,14 +1033,21 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                         }
                     }
                 } else {
-                    double acc[MAXM][5];
+                    double acc[MAXM][MAXV];
                     for (int i = 0; i < M; ++i)
                         for (int h = 0; h < V; ++h) acc[i][h] = 0.0;
                     for (int i = 0; i < M; ++i)
                         for (int m = 0; m < M; ++m) {
                             if (m == i) continue;
-                            double F[5];
-                            orc_two_point(c, c->volume, d, ul[i], ul[m], F);
+                            double F[MAXV];
+                            phys_two_point(c, c->volume, d, ul[i], ul[m], F);
+                            if (mhd) {
+                                /* split form of the Powell/GLM terms:
+                                 * sum_m S_im Phi(u_i) {{B_n, psi}}_im */
+                                double G[MAXV];
+                                phys_nc(c, d, ul[i], ul[i], ul[m], G);
+                                for (int h = 0; h < V; ++h) F[h] = F[h] + G[h];
+                            }
                             for (int h = 0; h < V; ++h) acc[i][h] += op->S[i * M + m] * F[h];
                         }
                     for (int i = 0; i < M; ++i) {
@@ -620,12 +1062,12 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                     }
                 }
                 if (do_fv) {
-                    double Fi[MAXM + 1][5];
+                    double Fi[MAXM + 1][MAXV];
                     for (int h = 0; h < V; ++h) {
                         Fi[0][h] = fm[h];
                         Fi[M][h] = fp[h];
                     }
-                    for (int l = 0; l < M - 1; ++l) orc_riemann(c, c->surface, d, ul[l], ul[l + 1], Fi[l + 1]);
+                    for (int l = 0; l < M - 1; ++l) phys_riemann(c, c->surface, d, ul[l], ul[l + 1], Fi[l + 1]);
                     for (int i = 0; i < M; ++i) {
                         int n = node_index(M, d, i, a, b);
                         for (int h = 0; h < V; ++h) fvr[d][h][n] = -jac * (Fi[i + 1][h] - Fi[i][h]) / op->w[i];
@@ -643,6 +1085,8 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                 out[h * M3 + n] = dg;
             }
         }
+    if (mhd) /* GLM damping (kernel/mcfv/3d/kernel_flux_mod.f90:496-497) */
+        for (int n = 0; n < M3; ++n) out[GLMP * M3 + n] = out[GLMP * M3 + n] - c->glm_alpha * ue[GLMP * M3 + n];
 }
 
 void orc_rhs(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, const double *u,
@@ -670,10 +1114,10 @@ double orc_calc_timestep(const orc_cfg *c, int64_t nelem, const double *delta, c
         const double *ue = u + e * V * M3;
         double lm[3] = {0.0, 0.0, 0.0}, vm = 0.0;
         for (int n = 0; n < M3; ++n) {
-            double uu[5];
-            for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
+            double uu[MAXV];
+            for (int h = 0; h < V; ++h) uu[h] = ue[h * M3 + n];
             for (int d = 0; d < 3; ++d) {
-                double l = orc_lambda_max(c, d, uu);
+                double l = phys_lambda_max(c, d, uu);
                 if (l > lm[d]) lm[d] = l;
                 double v = fabs(uu[1 + d] / uu[DENS]);
                 if (v > vm) vm = v;
@@ -703,44 +1147,48 @@ double orc_calc_timestep(const orc_cfg *c, int64_t nelem, const double *delta, c
  * whole element towards its mean is used (the reference's disabled
  * alternative, kernel_validate_mod.f90:191-210, with repair_decr steps).
  * Returns 1 if the element is valid afterwards. */
-static int elem_valid(double kappa, const double *ue, int M3) {
+static int state_valid(const orc_cfg *c, const double *uu) {
+    return (uu[DENS] > 0.0) && (phys_pressure(c, uu) > 0.0);
+}
+
+static int elem_valid(const orc_cfg *c, const double *ue, int M3) {
+    const int V = orc_nvars(c);
     for (int n = 0; n < M3; ++n) {
-        double uu[5];
-        for (int h = 0; h < 5; ++h) uu[h] = ue[h * M3 + n];
-        if (!((uu[DENS] > 0.0) && (pres5(kappa, uu) > 0.0))) return 0;
+        double uu[MAXV];
+        for (int h = 0; h < V; ++h) uu[h] = ue[h * M3 + n];
+        if (!state_valid(c, uu)) return 0;
     }
     return 1;
 }
 
 static int repair_element(const orc_cfg *c, const orc_ops *op, double *ue) {
-    const int M = op->M, M3 = op->M3;
-    const double kappa = c->gamma;
+    const int M = op->M, M3 = op->M3, V = orc_nvars(c);
     const double repair_decr = 0.01;
-    if (elem_valid(kappa, ue, M3)) return 1;
+    if (elem_valid(c, ue, M3)) return 1;
     int nlevels = 0;
     while ((1 << nlevels) < M) ++nlevels;
     int pow2 = ((1 << nlevels) == M);
     if (!pow2) {
-        double means[5] = {0, 0, 0, 0, 0}, wsum = 0.0;
+        double means[MAXV] = {0}, wsum = 0.0;
         for (int k = 0; k < M; ++k)
             for (int j = 0; j < M; ++j)
                 for (int i = 0; i < M; ++i) {
                     double ww = op->w[i] * op->w[j] * op->w[k];
                     wsum += ww;
-                    for (int h = 0; h < 5; ++h) means[h] += ww * ue[h * M3 + i + M * j + M * M * k];
+                    for (int h = 0; h < V; ++h) means[h] += ww * ue[h * M3 + i + M * j + M * M * k];
                 }
-        for (int h = 0; h < 5; ++h) means[h] = means[h] / wsum;
+        for (int h = 0; h < V; ++h) means[h] = means[h] / wsum;
         double squeeze = 1.0;
-        double orig[5 * MAXM * MAXM * MAXM];
-        memcpy(orig, ue, sizeof(double) * 5 * M3);
+        double orig[MAXV * MAXM * MAXM * MAXM];
+        memcpy(orig, ue, sizeof(double) * V * M3);
         while (squeeze > 0.0) {
             squeeze = squeeze - repair_decr;
             if (squeeze < 0.0) squeeze = 0.0;
-            for (int h = 0; h < 5; ++h)
+            for (int h = 0; h < V; ++h)
                 for (int n = 0; n < M3; ++n) ue[h * M3 + n] = means[h] + squeeze * (orig[h * M3 + n] - means[h]);
-            if (elem_valid(kappa, ue, M3)) break;
+            if (elem_valid(c, ue, M3)) break;
         }
-        return elem_valid(kappa, ue, M3);
+        return elem_valid(c, ue, M3);
     }
     for (int level = 1; level <= nlevels; ++level) {
         int mm = 1 << (level - 1);
@@ -748,10 +1196,10 @@ static int repair_element(const orc_cfg *c, const orc_ops *op, double *ue) {
         for (int r = 0; r < mm; ++r)
             for (int p = 0; p < mm; ++p)
                 for (int q = 0; q < mm; ++q) {
-                    double tiles[8][5], tw[8];
+                    double tiles[8][MAXV], tw[8];
                     for (int t = 0; t < 8; ++t) {
                         tw[t] = 0.0;
-                        for (int h = 0; h < 5; ++h) tiles[t][h] = 0.0;
+                        for (int h = 0; h < V; ++h) tiles[t][h] = 0.0;
                     }
                     for (int kk = 0; kk < 2; ++kk)
                         for (int jj = 0; jj < 2; ++jj)
@@ -762,47 +1210,47 @@ static int repair_element(const orc_cfg *c, const orc_ops *op, double *ue) {
                                         for (int i = (2 * q + ii) * nn; i < (2 * q + ii + 1) * nn; ++i) {
                                             double ww = op->w[i] * op->w[j] * op->w[k];
                                             tw[t] += ww;
-                                            for (int h = 0; h < 5; ++h)
+                                            for (int h = 0; h < V; ++h)
                                                 tiles[t][h] += ww * ue[h * M3 + i + M * j + M * M * k];
                                         }
-                                for (int h = 0; h < 5; ++h) tiles[t][h] = tiles[t][h] / tw[t];
+                                for (int h = 0; h < V; ++h) tiles[t][h] = tiles[t][h] / tw[t];
                             }
                     int ok = 1;
                     for (int t = 0; t < 8; ++t)
-                        if (!((tiles[t][DENS] > 0.0) && (pres5(kappa, tiles[t]) > 0.0))) ok = 0;
+                        if (!state_valid(c, tiles[t])) ok = 0;
                     if (ok) continue;
-                    double means[5], twsum = 0.0;
+                    double means[MAXV], twsum = 0.0;
                     for (int t = 0; t < 8; ++t) twsum += tw[t];
-                    for (int h = 0; h < 5; ++h) {
+                    for (int h = 0; h < V; ++h) {
                         double s = 0.0;
                         for (int t = 0; t < 8; ++t) s += tw[t] * tiles[t][h];
                         means[h] = s / twsum;
                     }
-                    double hiles[8][5];
+                    double hiles[8][MAXV];
                     for (int t = 0; t < 8; ++t)
-                        for (int h = 0; h < 5; ++h) hiles[t][h] = tiles[t][h] - means[h];
+                        for (int h = 0; h < V; ++h) hiles[t][h] = tiles[t][h] - means[h];
                     double squeeze = 1.0;
                     while (squeeze > 0.0) {
                         squeeze = squeeze - repair_decr;
                         if (squeeze < 0.0) squeeze = 0.0;
                         int okt = 1;
                         for (int t = 0; t < 8; ++t) {
-                            double tt[5];
-                            for (int h = 0; h < 5; ++h) tt[h] = means[h] + squeeze * hiles[t][h];
-                            if (!((tt[DENS] > 0.0) && (pres5(kappa, tt) > 0.0))) okt = 0;
+                            double tt[MAXV];
+                            for (int h = 0; h < V; ++h) tt[h] = means[h] + squeeze * hiles[t][h];
+                            if (!state_valid(c, tt)) okt = 0;
                         }
                         if (okt) break;
                     }
                     for (int k = 2 * r * nn; k < (2 * r + 2) * nn; ++k)
                         for (int j = 2 * p * nn; j < (2 * p + 2) * nn; ++j)
                             for (int i = 2 * q * nn; i < (2 * q + 2) * nn; ++i)
-                                for (int h = 0; h < 5; ++h) {
+                                for (int h = 0; h < V; ++h) {
                                     double *x = &ue[h * M3 + i + M * j + M * M * k];
                                     *x = means[h] + squeeze * (*x - means[h]);
                                 }
                 }
     }
-    return elem_valid(kappa, ue, M3);
+    return elem_valid(c, ue, M3);
 }
 
 int orc_repair(const orc_cfg *c, double *ue) {
@@ -924,10 +1372,18 @@ int orc_run(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *d
     double *r0 = (double *)calloc(total, sizeof(double));
     double *r1 = (double *)calloc(total, sizeof(double));
     int rc = 0;
+    orc_cfg cc = *c;
     for (int it = 0; it < nsteps; ++it) {
-        double dt = orc_calc_timestep(c, nelem, delta, u, NULL, NULL);
+        double lmax, vmax;
+        double dt = orc_calc_timestep(c, nelem, delta, u, &lmax, &vmax);
+        if (c->eq == ORC_EQ_MHD) {
+            /* glm_ch = 2|lmax - vmax|, glm_alpha = glm_ch/0.18
+             * (kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:55-60) */
+            cc.glm_ch = 2.0 * fabs(lmax - vmax);
+            cc.glm_alpha = cc.glm_ch / 0.18;
+        }
         if (dt_hist) dt_hist[it] = dt;
-        rc = orc_step(c, nelem, nbr, delta, u, r0, r1, dt);
+        rc = orc_step(&cc, nelem, nbr, delta, u, r0, r1, dt);
         if (rc) break;
     }
     free(r0);

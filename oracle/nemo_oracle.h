@@ -70,6 +70,22 @@ void orc_cons2evec(const orc_cfg *c, const double *u, double *w);
 double orc_entropy(const orc_cfg *c, const double *u);
 int orc_isvalid(const orc_cfg *c, const double *u);
 
+/* ---- pointwise physics, dispatched on cfg->eq (Euler or GLM-MHD) ------ */
+/* GLM-MHD state: rho, rho*v[3], E, B[3], psi (equations/mhd/equations_const_mod.f90). */
+double orc_mhd_pressure(const orc_cfg *c, const double *u);
+double orc_mhd_fastspeed(const orc_cfg *c, const double *u);   /* x direction */
+void orc_phys_flux(const orc_cfg *c, int dir, const double *u, double *f);
+void orc_phys_riemann(const orc_cfg *c, int kind, int dir, const double *uL,
+                      const double *uR, double *f);
+void orc_phys_two_point(const orc_cfg *c, int kind, int dir, const double *u,
+                        const double *v, double *f);
+/* Powell/GLM non-conservative two-point term Phi(uI) * {{B_n, psi}}(uL, uR)
+ * (equations/mhd/non_conservative_flux_mod.f90); zero for Euler. */
+void orc_phys_nc(const orc_cfg *c, int dir, const double *uI, const double *uL,
+                 const double *uR, double *f);
+double orc_phys_lambda_max(const orc_cfg *c, int dir, const double *u);
+int orc_phys_isvalid(const orc_cfg *c, const double *u);
+
 /* ---- element operators -------------------------------------------------- */
 int orc_nvars(const orc_cfg *c);
 void orc_indicator(const orc_cfg *c, int64_t nelem, const double *u,

@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libnemo_oracle.so")
 _REF_A = os.path.join(_HERE, "_ref", "libnemo_ref_a.so")
 _REF_B = os.path.join(_HERE, "_ref", "libnemo_ref_b.so")
+_REF_C = os.path.join(_HERE, "_ref", "libnemo_ref_c.so")
 
 EQ_EULER, EQ_MHD = 0, 1
 VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
@@ -92,6 +93,10 @@ def lib():
         L.orc_run.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, ctypes.c_int, _P]
         L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_double, _P, _P]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_mhd_pressure.restype = ctypes.c_double
+        L.orc_mhd_fastspeed.restype = ctypes.c_double
+        L.orc_phys_lambda_max.restype = ctypes.c_double
+        L.orc_phys_isvalid.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -179,6 +184,56 @@ def entropy(cfg: Cfg, u) -> float:
 def isvalid(cfg: Cfg, u) -> bool:
     u = np.ascontiguousarray(u, dtype=np.float64)
     return bool(lib().orc_isvalid(ctypes.byref(cfg), _ptr(u)))
+
+
+# ---- pointwise, dispatched on cfg.eq (Euler: 5 vars, GLM-MHD: 9) ----------
+
+def _vec(u):
+    return np.ascontiguousarray(u, dtype=np.float64)
+
+
+def mhd_pressure(cfg: Cfg, u) -> float:
+    return lib().orc_mhd_pressure(ctypes.byref(cfg), _ptr(_vec(u)))
+
+
+def mhd_fastspeed(cfg: Cfg, u) -> float:
+    return lib().orc_mhd_fastspeed(ctypes.byref(cfg), _ptr(_vec(u)))
+
+
+def phys_flux(cfg: Cfg, d: int, u) -> np.ndarray:
+    u = _vec(u)
+    f = np.zeros_like(u)
+    lib().orc_phys_flux(ctypes.byref(cfg), ctypes.c_int(d), _ptr(u), _ptr(f))
+    return f
+
+
+def phys_riemann(cfg: Cfg, kind: int, d: int, uL, uR) -> np.ndarray:
+    uL, uR = _vec(uL), _vec(uR)
+    f = np.zeros_like(uL)
+    lib().orc_phys_riemann(ctypes.byref(cfg), ctypes.c_int(kind), ctypes.c_int(d), _ptr(uL), _ptr(uR), _ptr(f))
+    return f
+
+
+def phys_two_point(cfg: Cfg, kind: int, d: int, u, v) -> np.ndarray:
+    u, v = _vec(u), _vec(v)
+    f = np.zeros_like(u)
+    lib().orc_phys_two_point(ctypes.byref(cfg), ctypes.c_int(kind), ctypes.c_int(d), _ptr(u), _ptr(v), _ptr(f))
+    return f
+
+
+def phys_nc(cfg: Cfg, d: int, uI, uL, uR) -> np.ndarray:
+    uI, uL, uR = _vec(uI), _vec(uL), _vec(uR)
+    f = np.zeros_like(uI)
+    lib().orc_phys_nc(ctypes.byref(cfg), ctypes.c_int(d), _ptr(uI), _ptr(uL), _ptr(uR), _ptr(f))
+    return f
+
+
+def phys_lambda_max(cfg: Cfg, d: int, u) -> float:
+    return lib().orc_phys_lambda_max(ctypes.byref(cfg), ctypes.c_int(d), _ptr(_vec(u)))
+
+
+def phys_isvalid(cfg: Cfg, u) -> bool:
+    return bool(lib().orc_phys_isvalid(ctypes.byref(cfg), _ptr(_vec(u))))
 
 
 # ---- element operators ---------------------------------------------------
@@ -370,3 +425,60 @@ class Ref:
 
 def ref_available() -> bool:
     return os.path.exists(_REF_A) and os.path.exists(_REF_B)
+
+
+class RefMHD:
+    """The reference's GLM-MHD modules compiled in place (oracle/_ref, library
+    C: equations/mhd/{equations_mod, riemann/rusanov, non_conservative_flux_mod}
+    with the Orszag-Tang setup_config_mod, kappa = 5/3, mu0 = 1)."""
+
+    def __init__(self):
+        if not os.path.exists(_REF_C):
+            raise FileNotFoundError("oracle/_ref not built (make -C oracle ref)")
+        c = self.c = ctypes.CDLL(_REF_C)
+        c.ref_mhd_set_glm.argtypes = [ctypes.c_double]
+        c.ref_mhd_pressure.restype = ctypes.c_double
+        c.ref_mhd_pressure.argtypes = [_P]
+        c.ref_mhd_flux.argtypes = [ctypes.c_int, _P, _P]
+        c.ref_mhd_lambda_max.restype = ctypes.c_double
+        c.ref_mhd_lambda_max.argtypes = [ctypes.c_int, _P]
+        c.ref_mhd_fastspeed.restype = ctypes.c_double
+        c.ref_mhd_fastspeed.argtypes = [_P]
+        c.ref_mhd_isvalid.restype = ctypes.c_int
+        c.ref_mhd_isvalid.argtypes = [_P]
+        c.ref_mhd_rusanov.argtypes = [ctypes.c_int, _P, _P, _P]
+        c.ref_mhd_nc.argtypes = [ctypes.c_int, _P, _P, _P, _P]
+
+    def set_glm(self, ch):
+        self.c.ref_mhd_set_glm(float(ch))
+
+    def pressure(self, u):
+        return self.c.ref_mhd_pressure(_ptr(_vec(u)))
+
+    def flux(self, d, u):
+        f = np.zeros(9)
+        self.c.ref_mhd_flux(d, _ptr(_vec(u)), _ptr(f))
+        return f
+
+    def lambda_max(self, d, u):
+        return self.c.ref_mhd_lambda_max(d, _ptr(_vec(u)))
+
+    def fastspeed(self, u):
+        return self.c.ref_mhd_fastspeed(_ptr(_vec(u)))
+
+    def isvalid(self, u):
+        return bool(self.c.ref_mhd_isvalid(_ptr(_vec(u))))
+
+    def rusanov(self, d, uL, uR):
+        f = np.zeros(9)
+        self.c.ref_mhd_rusanov(d, _ptr(_vec(uL)), _ptr(_vec(uR)), _ptr(f))
+        return f
+
+    def nc(self, d, uI, uL, uR):
+        f = np.zeros(9)
+        self.c.ref_mhd_nc(d, _ptr(_vec(uI)), _ptr(_vec(uL)), _ptr(_vec(uR)), _ptr(f))
+        return f
+
+
+def ref_mhd_available() -> bool:
+    return os.path.exists(_REF_C)

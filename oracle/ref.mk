@@ -33,3 +33,21 @@ _ref/libnemo_ref_b.so: ref_shim_b.f90 ref.mk
 	  case $$f in /*) src=$$f;; *) src=../../$$f;; esac; \
 	  $(FC) $(FFLAGS) -c $$src -o $$(basename $$f .f90).o || exit 1; done
 	$(FC) -shared -Wl,-Bsymbolic -o $@ _ref/b/*.o
+
+# Library C: GLM-MHD (PP_N_VARS=9) with the Orszag-Tang setup's constants.
+FFLAGS_MHD := -cpp -O2 -fPIC -ffp-contract=off -DPP_N_NODES=4 -DPP_N_DIMS=3 -DPP_N_VARS=9 -DPP_EQN_MHD=1
+REF_C := $(REF)/utils/constants_mod.f90 \
+         $(REF)/setups/mhd/3d/orszag-tang/setup_config_mod.f90 \
+         $(REF)/equations/mhd/equations_const_mod.f90 \
+         $(REF)/equations/mhd/equations_share_mod.f90 \
+         $(REF)/equations/mhd/equations_mod.f90 \
+         $(REF)/equations/mhd/riemann/rusanov/riemann_mod.f90 \
+         $(REF)/equations/mhd/non_conservative_flux_mod.f90 \
+         ref_shim_c.f90
+
+_ref/libnemo_ref_c.so: ref_shim_c.f90 ref.mk
+	rm -rf _ref/c && mkdir -p _ref/c
+	cd _ref/c && for f in $(REF_C); do \
+	  case $$f in /*) src=$$f;; *) src=../../$$f;; esac; \
+	  $(FC) $(FFLAGS_MHD) -c $$src -o $$(basename $$f .f90).o 2>/dev/null || exit 1; done
+	$(FC) -shared -Wl,-Bsymbolic -o $@ _ref/c/*.o

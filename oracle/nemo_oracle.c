@@ -1068,9 +1068,23 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                         Fi[M][h] = fp[h];
                     }
                     for (int l = 0; l < M - 1; ++l) phys_riemann(c, c->surface, d, ul[l], ul[l + 1], Fi[l + 1]);
+                    /* Powell/GLM terms at the interior subcell interfaces:
+                     * cell l takes Phi(u_l) {{B_n, psi}}_{l+1/2} on its right
+                     * face and cell l+1 takes Phi(u_{l+1}) {{B_n, psi}}_{l+1/2}
+                     * on its left face (kernel/mcfv/3d/kernel_flux_mod.f90:
+                     * 478-494); the element faces already carry theirs. */
+                    double Nr[MAXM][MAXV], Nl[MAXM][MAXV];
+                    for (int i = 0; i < M; ++i)
+                        for (int h = 0; h < V; ++h) Nr[i][h] = Nl[i][h] = 0.0;
+                    if (mhd)
+                        for (int l = 0; l < M - 1; ++l) {
+                            phys_nc(c, d, ul[l], ul[l], ul[l + 1], Nr[l]);
+                            phys_nc(c, d, ul[l + 1], ul[l], ul[l + 1], Nl[l + 1]);
+                        }
                     for (int i = 0; i < M; ++i) {
                         int n = node_index(M, d, i, a, b);
-                        for (int h = 0; h < V; ++h) fvr[d][h][n] = -jac * (Fi[i + 1][h] - Fi[i][h]) / op->w[i];
+                        for (int h = 0; h < V; ++h)
+                            fvr[d][h][n] = -jac * ((Fi[i + 1][h] + Nr[i][h]) - (Fi[i][h] + Nl[i][h])) / op->w[i];
                     }
                 }
             }

@@ -342,3 +342,43 @@ def test_cfl_closed_form():
     assert lmax == max(lam)
     assert vmax == 0.5
     assert np.isclose(dt, 0.5 * 0.5 / (2 * (N + 1) - 1) * min(m.delta[0, d] / lam[d] for d in range(3)), rtol=1e-15)
+
+
+def test_fv_subcell_operator_with_nonconservative_terms():
+    """alpha = 1, data varying only in x with a non-solenoidal B_x: the
+    blended operator is the first-order FV update on the LGL sub-grid with
+    the Powell/GLM terms taken per side of every subcell interface
+    (kernel/mcfv/3d/kernel_flux_mod.f90:470-497), built here in 1-D from the
+    pointwise functions."""
+    N, M = 3, 4
+    m = meshmod.uniform_periodic(2)
+    x, w = O.lgl(N)
+    rng = np.random.default_rng(12)
+    # per x-node states, shared by every (y, z) line: element ex = e % 2 in x
+    line = {ex: [prim2cons(1 + 0.2 * rng.uniform(-1, 1), rng.uniform(-0.3, 0.3, 3), 1 + 0.2 * rng.uniform(-1, 1),
+                           [0.5 + 0.3 * rng.uniform(-1, 1), 0.4, -0.2], 0.1 * rng.uniform(-1, 1)) for _ in range(M)]
+            for ex in range(2)}
+    u = np.zeros((m.nelem, 9, M ** 3))
+    for e in range(m.nelem):
+        ex = int(m.lower[e, 0] / m.delta[e, 0] + 0.5)
+        for n in range(M ** 3):
+            u[e, :, n] = line[ex][n % M]
+    glm_alpha = 2.0
+    cfg = _cfg(surface=O.SURF_HLLD, indicator=O.IND_PP, glm_ch=0.9, glm_alpha=glm_alpha)
+    r = O.rhs(cfg, m.nbr, m.delta, u, np.ones(m.nelem))
+    for e in range(m.nelem):
+        ex = int(m.lower[e, 0] / m.delta[e, 0] + 0.5)
+        ul, uext_m, uext_p = line[ex], line[1 - ex][M - 1], line[1 - ex][0]
+        dx = m.delta[e, 0]
+        F = [O.phys_riemann(cfg, O.SURF_HLLD, 0, uext_m, ul[0])]
+        F += [O.phys_riemann(cfg, O.SURF_HLLD, 0, ul[i], ul[i + 1]) for i in range(M - 1)]
+        F += [O.phys_riemann(cfg, O.SURF_HLLD, 0, ul[M - 1], uext_p)]
+        left = [uext_m] + ul[:-1]
+        right = ul[1:] + [uext_p]
+        for i in range(M):
+            fr = F[i + 1] + O.phys_nc(cfg, 0, ul[i], ul[i], right[i])
+            fl = F[i] + O.phys_nc(cfg, 0, ul[i], left[i], ul[i])
+            expect = -(2.0 / dx) * (fr - fl) / w[i]
+            expect[8] -= glm_alpha * ul[i][8]
+            for n in range(i, M ** 3, M):
+                assert np.allclose(r[e, :, n], expect, rtol=1e-12, atol=1e-11), (e, i)

@@ -49,6 +49,7 @@ extern "C" {
 /* surface Riemann solver (riemann_mod slots, scripts/nemo.py:398-415) */
 #define NEMO_SURF_RUSANOV 0    /* equations/euler/riemann/rusanov */
 #define NEMO_SURF_HLLC 1       /* equations/euler/riemann/hllc (HLLC/HLLE hybrid) */
+#define NEMO_SURF_HLLD 2       /* equations/mhd/riemann/hlld, GLM-coupled (GLM-MHD) */
 /* troubled-cell indicator for DG/FV blending */
 #define NEMO_IND_NONE 0        /* pure DG */
 #define NEMO_IND_PP 1          /* Persson-Peraire on rho*p */
@@ -74,7 +75,7 @@ typedef struct nemo_gpu_config {
     double alpha_max;      /* blending factor clamp */
     double alpha_min;      /* blending factors below this become 0 */
     double cfl;            /* setup_config_mod::CFL */
-    double mu0;            /* MHD only */
+    double mu0;            /* GLM-MHD only: setup_config_mod::mu0 */
 } nemo_gpu_config;
 
 /* Create / destroy a solver (replaces hook_init_400 operator setup,
@@ -105,6 +106,16 @@ int nemo_gpu_indicator(void *handle, double *alpha);
  * 0.5*CFL/(2*N_NODES-1)*min(delta/lambda); lmax and vmax are the global
  * maxima used for GLM cleaning.  Synchronises. */
 int nemo_gpu_calc_timestep(void *handle, double *dt, double *lmax, double *vmax);
+
+/* GLM divergence cleaning (GLM-MHD): the cleaning speed c_h and damping
+ * alpha that the operator uses, the reference's module variables
+ * equations_share_mod::glm_ch / glm_alpha.  nemo_gpu_calc_timestep and
+ * nemo_gpu_run set them from the CFL reduction as the reference's timestep
+ * hook does (c_h = 2|lmax - vmax|, alpha = c_h/0.18,
+ * kernel/mcfv/3d/kernel_timestep_mod.f90:53-56); nemo_gpu_set_glm assigns
+ * them directly. */
+int nemo_gpu_set_glm(void *handle, double ch, double alpha);
+int nemo_gpu_get_glm(void *handle, double *ch, double *alpha);
 
 /* One RK stage s = 1..5 with time step dt (replaces hook_mid_rk_timestep_500
  * -> rkstep_cb -> kernel_evolve + timedisc_probe + kernel_repair,

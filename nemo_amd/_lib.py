@@ -17,7 +17,7 @@ NEMO_ERR_NOT_READY = 4
 
 EQ_EULER, EQ_GLM_MHD = 0, 1
 VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
-SURF_RUSANOV, SURF_HLLC = 0, 1
+SURF_RUSANOV, SURF_HLLC, SURF_HLLD = 0, 1, 2
 IND_NONE, IND_PP = 0, 1
 TD_LSRK54, TD_SSPRK54 = 0, 1
 
@@ -63,6 +63,8 @@ SIGNATURES = {
     "nemo_gpu_rhs": [_VP, _P],
     "nemo_gpu_indicator": [_VP, _P],
     "nemo_gpu_calc_timestep": [_VP, _P, _P, _P],
+    "nemo_gpu_set_glm": [_VP, ctypes.c_double, ctypes.c_double],
+    "nemo_gpu_get_glm": [_VP, _P, _P],
     "nemo_gpu_stage": [_VP, ctypes.c_int32, ctypes.c_double],
     "nemo_gpu_step": [_VP, ctypes.c_double],
     "nemo_gpu_run": [_VP, ctypes.c_int32, _P],

@@ -26,288 +26,26 @@
 // separated by barriers.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include <type_traits>
 
-#include "physics.hpp"
+#include "dg_common.hpp"
+#include "dg_euler.hpp"
+#include "dg_mhd.hpp"
 
 namespace nemo {
-
-enum { VOL_WEAK = 0, VOL_CENTRAL = 1, VOL_EC = 2 };
-enum { SURF_RUSANOV = 0, SURF_HLLC = 1 };
-enum { TD_LSRK54 = 0, TD_SSPRK54 = 1 };
-enum { IND_NONE = 0, IND_PP = 1 };
-
-// Operator tables for M LGL nodes (built on the host, passed by value).
-template <int M, typename Real>
-struct OpTable {
-    Real w[M], inv_w[M];
-    Real S[M][M];       // split-form matrix 2D - W^-1 B (zero diagonal)
-    Real dgDiff[M][M];  // weak-form volume matrix, reference scaling
-    Real lagrM[M], lagrP[M];
-    Real Vinv[M][M];    // nodal -> modal (normalised Legendre)
-};
-
-struct RKCoeffs {
-    double lsrk_a[5], lsrk_c[5];
-    double ssp[23];  // A0..A3, B0..B3, C0..C3, D0..D3, E0..E6
-    double w64[8];   // FP64 quadrature weights (repair means)
-};
-
-template <int M, typename Real>
-struct StageArgs {
-    const double* __restrict__ u_in;
-    double* __restrict__ u_out;
-    double* __restrict__ r0;
-    double* __restrict__ r1;
-    double* __restrict__ rhs_out;    // non-null: write the RHS, no update
-    double* __restrict__ alpha_out;  // optional per-element blending factor
-    const int32_t* __restrict__ nbr; // [nelem][6]
-    const double* __restrict__ delta;// [nelem][3]
-    const double* __restrict__ dt_dev;
-    int* __restrict__ status;
-    double* __restrict__ cfl_part;   // non-null: per-block CFL partials of the new state
-    long long nelem;
-    double dt;
-    int stage;      // 1..5
-    int timedisc;
-    int indicator;
-    int repair;
-    double alpha_max, alpha_min, ind_threshold, ind_s;
-    Gas<Real> gas;
-    Gas<double> gas64;  // FP64 constants (CFL, FP64 pressures of the FP32 operator)
-    OpTable<M, Real> op;
-    RKCoeffs rk;
-};
-
-template <int D, int M>
-__device__ __forceinline__ constexpr int nidx(int l, int a, int b) {
-    return D == 0 ? l + M * a + M * M * b : (D == 1 ? a + M * l + M * M * b : a + M * b + M * M * l);
-}
-
-// LDS slot of node n = i + M*j + M*M*k inside an element.  For M = 4 the
-// slot is 16k + 4(i^j) + (i^k): a bijection under which the 64 lanes of any
-// direction pass (one node per line, 16 lines x 4 elements) read 16 distinct
-// 8-byte bank pairs, 4 lanes each -- the minimum of 4 LDS passes for a
-// 64-lane b64 access.  Without it the x- and y-passes hit only 4 bank pairs
-// (16-way conflicts).  Other M keep the natural order.
-template <int M>
-__device__ __forceinline__ constexpr int swz(int n) {
-    if constexpr (M == 4) {
-        const int i = n & 3, j = (n >> 2) & 3, k = n >> 4;
-        return 16 * k + 4 * (i ^ j) + (i ^ k);
-    } else {
-        return n;
-    }
-}
-
-// LDS slot of block-local node index idx = element*M^3 + n.
-template <int M>
-__device__ __forceinline__ constexpr int slot(int idx) {
-    constexpr int M3 = M * M * M;
-    return (idx / M3) * M3 + swz<M>(idx % M3);
-}
-
-template <int SURF, int D, typename Real>
-__device__ __forceinline__ void riemann(const Gas<Real>& g, const NodeState<Real>& uL, const NodeState<Real>& uR,
-                                        Real* f) {
-    if constexpr (SURF == SURF_HLLC)
-        riemann_hllc<D>(g, uL, uR, f);
-    else
-        riemann_rusanov<D>(g, uL, uR, f);
-}
-
-// Block index -> element block, so that each XCD (blocks are dealt to the 8
-// XCDs round-robin) walks a contiguous stretch of the element order and finds
-// face neighbours in its own L2.
-__device__ __forceinline__ long long xcd_block(long long nblocks) {
-    const long long bid = blockIdx.x;
-    if ((nblocks & 7) != 0) return bid;
-    const long long per = nblocks >> 3;
-    return (bid & 7) * per + (bid >> 3);
-}
-
-template <int M, int BE>
-struct StageShape {
-    static constexpr int M2 = M * M, M3 = M2 * M;
-    static constexpr int NT = BE * M2;  // threads per workgroup
-    static constexpr int NN = BE * M3;  // nodes per workgroup
-};
-
-// One direction pass: contributions of direction D to the node accumulators.
-// The thread's M line nodes (and the two neighbour face nodes) become node
-// states once; the surface Riemann problems, the M(M-1)/2 two-point fluxes
-// and the M-1 FV interface fluxes all read those states.
-template <int D, int M, int BE, int VOL, int SURF, typename Real>
-__device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, double (*s_u)[BE * M * M * M],
-                                               Real (*s_acc)[BE * M * M * M], const Real* s_alpha,
-                                               const double* s_p64) {
-    using Sh = StageShape<M, BE>;
-    constexpr int M2 = Sh::M2, M3 = Sh::M3;
-    constexpr bool BETA = (VOL == VOL_EC);
-    const int tid = threadIdx.x;
-    const int el = tid / M2;
-    const int line = tid - el * M2;
-    const int ta = line % M, tb = line / M;
-    const long long e = e0 + el;
-    const bool valid = e < a.nelem;
-    const long long EM3 = a.nelem * (long long)M3;
-    const Gas<Real>& g = a.gas;
-    const int base = el * M3;
-
-    constexpr bool F32 = sizeof(Real) == 4;
-    NodeState<Real> S[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
-#pragma unroll
-        for (int h = 0; h < 5; ++h) S[i].u[h] = Real(s_u[h][n]);
-        make_state<D, BETA>(g, S[i]);
-        if constexpr (F32) S[i].p64 = s_p64[n];
-    }
-    Real jac = Real(0), sdx = Real(0);
-    Real fm[5], fp[5];
-    {
-        NodeState<Real> Nm, Np;
-        if (valid) {
-            const long long em = a.nbr[e * 6 + 2 * D];
-            const long long ep = a.nbr[e * 6 + 2 * D + 1];
-            const double dx = a.delta[e * 3 + D];
-            jac = Real(2.0 / dx);
-            sdx = Real(double(M) / dx);
-            const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
-            const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
-            double dm[5], dp[5];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                dm[h] = um[h * EM3];
-                dp[h] = up[h * EM3];
-                Nm.u[h] = Real(dm[h]);
-                Np.u[h] = Real(dp[h]);
-            }
-            if constexpr (F32) {
-                Nm.p64 = pressure(a.gas64, dm);
-                Np.p64 = pressure(a.gas64, dp);
-            }
-        } else {
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                Nm.u[h] = S[0].u[h];
-                Np.u[h] = S[M - 1].u[h];
-            }
-            Nm.p64 = S[0].p64;
-            Np.p64 = S[M - 1].p64;
-        }
-        make_state<D, false>(g, Nm);
-        make_state<D, false>(g, Np);
-        riemann<SURF, D>(g, Nm, S[0], fm);
-        riemann<SURF, D>(g, S[M - 1], Np, fp);
-    }
-
-    Real c[M][5];
-    if constexpr (VOL == VOL_WEAK) {
-        // 3-D LGL extension of kernel/dgsem/weakform/2d/kernel_flux_mod.f90:242-277:
-        // rhs_i = lagrM_i*(sdx f*_-) - lagrP_i*(sdx f*_+) + sum_j dgDiff_ij (sdx f_j)
-        Real fx[M][5];
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            flux_s<D>(S[j], fx[j]);
-#pragma unroll
-            for (int h = 0; h < 5; ++h) fx[j][h] = sdx * fx[j][h];
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                Real s = Real(0);
-#pragma unroll
-                for (int j = 0; j < M; ++j) s += a.op.dgDiff[i][j] * fx[j][h];
-                Real surf = Real(0);
-                if (i == 0) surf = a.op.lagrM[0] * (sdx * fm[h]);
-                if (i == M - 1) surf = surf - a.op.lagrP[M - 1] * (sdx * fp[h]);
-                c[i][h] = surf + s;
-            }
-    } else {
-        // Split-form flux differencing (Gassner/Winters/Kopriva 2016):
-        // rhs_i = -jac [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ];
-        // each pair is evaluated once and used by both nodes (f# symmetric).
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) c[i][h] = Real(0);
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int m = i + 1; m < M; ++m) {
-                Real F[5];
-                if constexpr (VOL == VOL_EC)
-                    ec_flux<D>(g, S[i], S[m], F);
-                else
-                    central_flux<D>(S[i], S[m], F);
-#pragma unroll
-                for (int h = 0; h < 5; ++h) {
-                    c[i][h] += a.op.S[i][m] * F[h];
-                    c[m][h] += a.op.S[m][i] * F[h];
-                }
-            }
-#pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            c[0][h] = c[0][h] - fm[h] * a.op.inv_w[0];
-            c[M - 1][h] = c[M - 1][h] + fp[h] * a.op.inv_w[M - 1];
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) c[i][h] = -jac * c[i][h];
-    }
-
-    // FV subcells on the LGL sub-grid (Hennemann et al. 2021), first order:
-    // rhs_i = -jac (F_{i+1/2} - F_{i-1/2}) / w_i with the element-face
-    // fluxes f*_- and f*_+ at the ends; blended as (1-alpha) DG + alpha FV.
-    const Real alpha = s_alpha[el];
-    if (alpha > Real(0)) {
-        const Real beta = Real(1) - alpha;
-        Real Fprev[5];
-#pragma unroll
-        for (int h = 0; h < 5; ++h) Fprev[h] = fm[h];
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            Real Fnext[5];
-            if (i == M - 1) {
-#pragma unroll
-                for (int h = 0; h < 5; ++h) Fnext[h] = fp[h];
-            } else {
-                riemann<SURF, D>(g, S[i], S[i + 1], Fnext);
-            }
-            const Real sc = -jac * a.op.inv_w[i];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                c[i][h] = beta * c[i][h] + alpha * (sc * (Fnext[h] - Fprev[h]));
-                Fprev[h] = Fnext[h];
-            }
-        }
-    }
-
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
-#pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            if constexpr (D == 0)
-                s_acc[h][n] = c[i][h];
-            else
-                s_acc[h][n] += c[i][h];
-        }
-    }
-}
 
 // Persson-Peraire indicator on rho*p (Hennemann et al. 2021, sec. 3): modal
 // energies of the normalised-Legendre expansion (utils/interpol_mod.f90:
 // 274-311, 584-601), alpha from the logistic ramp around the threshold
 // 0.5*10^(-1.8*M^0.25), clipped by alpha_min/alpha_max.
-template <int M, int BE, typename Real>
-__device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long long e0, double (*s_u)[BE * M * M * M],
-                                             Real (*s_acc)[BE * M * M * M], Real* s_alpha, const double* s_p64) {
+//
+// Returns the blending factor of the calling thread's element (one
+// workgroup is one wavefront, so it is broadcast by a lane shuffle).
+template <int EQ, int M, int BE, typename Real, typename TU>
+__device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long long e0,
+                                             const TU (*s_u)[BE * M * M * M], Real (*s_acc)[BE * M * M * M],
+                                             const double* s_p64) {
+    constexpr int V = NVars<EQ>::V;
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
     const int tid = threadIdx.x;
@@ -315,15 +53,14 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
     Real* t1 = s_acc[0];
     Real* t2 = s_acc[1];
     for (int idx = tid; idx < NN; idx += NT) {
-        if constexpr (sizeof(Real) == 4) {
-            const int sl = slot<M>(idx);
-            q[sl] = Real(s_u[DENS][sl] * s_p64[sl]);
+        const int sl = slot<M>(idx);
+        if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
+            q[sl] = Real(double(s_u[DENS][sl]) * s_p64[sl]);
         } else {
-            const int sl = slot<M>(idx);
-            double uu[5];
+            double uu[V];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][sl];
-            q[sl] = uu[DENS] * pressure(a.gas64, uu);
+            for (int h = 0; h < V; ++h) uu[h] = s_u[h][sl];
+            q[sl] = Real(uu[DENS] * pressure64<EQ>(a.ph64, uu));
         }
     }
     __syncthreads();
@@ -377,6 +114,7 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
         s_acc[4][tid] = c2;
     }
     __syncthreads();
+    Real alpha_el = Real(0);
     if (line == 0) {
         Real tot = Real(0), c1 = Real(0), c2 = Real(0);
         for (int l = 0; l < M2; ++l) {
@@ -396,23 +134,27 @@ __device__ __forceinline__ void indicator_pp(const StageArgs<M, Real>& a, long l
         else if (alpha > 1.0 - a.alpha_min)
             alpha = 1.0;
         if (alpha > a.alpha_max) alpha = a.alpha_max;
-        s_alpha[el] = Real(alpha);
+        alpha_el = Real(alpha);
         const long long e = e0 + el;
         if (a.alpha_out && e < a.nelem) a.alpha_out[e] = alpha;
     }
-    __syncthreads();
+    alpha_el = __shfl(alpha_el, el * M2);
+    __syncthreads();  // the work rows are the direction passes' accumulators
+    return alpha_el;
 }
 
 // Positivity repair of one element held in LDS (one thread; rare path):
 // hierarchical 2x2x2 tile squeeze of kernel/mcfv/3d/kernel_validate_mod.f90:
 // 159-274 with quadrature-weighted tile means (LGL nodes) so the element
 // integral is kept.  Returns false if the element is still invalid.
-template <int M, typename Real>
-__device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* su, int base) {
+template <int EQ, int M>
+__device__ bool repair_element(const Phys64& P, const double* w, double* const* su, int base) {
+    using Real = double;
+    constexpr int V = NVars<EQ>::V;
     auto node_ok = [&](int n) {
-        Real uu[5];
-        for (int h = 0; h < 5; ++h) uu[h] = su[h][base + swz<M>(n)];
-        return isvalid(g, uu);
+        Real uu[V];
+        for (int h = 0; h < V; ++h) uu[h] = su[h][base + swz<M>(n)];
+        return isvalid64<EQ>(P, uu);
     };
     auto all_ok = [&]() {
         for (int n = 0; n < M * M * M; ++n)
@@ -424,15 +166,15 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
     int nlevels = 0;
     while ((1 << nlevels) < M) ++nlevels;
     if ((1 << nlevels) != M) {
-        Real means[5] = {0, 0, 0, 0, 0}, wsum = 0;
+        Real means[V] = {}, wsum = 0;
         for (int k = 0; k < M; ++k)
             for (int j = 0; j < M; ++j)
                 for (int i = 0; i < M; ++i) {
                     const Real ww = w[i] * w[j] * w[k];
                     wsum += ww;
-                    for (int h = 0; h < 5; ++h) means[h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
+                    for (int h = 0; h < V; ++h) means[h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
                 }
-        for (int h = 0; h < 5; ++h) means[h] = means[h] / wsum;
+        for (int h = 0; h < V; ++h) means[h] = means[h] / wsum;
         // squeeze all nodes towards the mean in repair_decr steps
         Real squeeze = Real(1);
         Real scale_prev = Real(1);
@@ -441,7 +183,7 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
             if (squeeze < Real(0)) squeeze = Real(0);
             const Real f = squeeze / scale_prev;  // rescale current deviations
             for (int n = 0; n < M * M * M; ++n)
-                for (int h = 0; h < 5; ++h) {
+                for (int h = 0; h < V; ++h) {
                     Real* x = &su[h][base + swz<M>(n)];
                     *x = means[h] + (squeeze == Real(0) ? Real(0) : f) * (*x - means[h]);
                 }
@@ -456,10 +198,10 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
         for (int r = 0; r < mm; ++r)
             for (int p = 0; p < mm; ++p)
                 for (int q = 0; q < mm; ++q) {
-                    Real tiles[8][5], tw[8];
+                    Real tiles[8][V], tw[8];
                     for (int t = 0; t < 8; ++t) {
                         tw[t] = Real(0);
-                        for (int h = 0; h < 5; ++h) tiles[t][h] = Real(0);
+                        for (int h = 0; h < V; ++h) tiles[t][h] = Real(0);
                     }
                     for (int kk = 0; kk < 2; ++kk)
                         for (int jj = 0; jj < 2; ++jj)
@@ -470,18 +212,18 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
                                         for (int i = (2 * q + ii) * nn; i < (2 * q + ii + 1) * nn; ++i) {
                                             const Real ww = w[i] * w[j] * w[k];
                                             tw[t] += ww;
-                                            for (int h = 0; h < 5; ++h)
+                                            for (int h = 0; h < V; ++h)
                                                 tiles[t][h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
                                         }
-                                for (int h = 0; h < 5; ++h) tiles[t][h] = tiles[t][h] / tw[t];
+                                for (int h = 0; h < V; ++h) tiles[t][h] = tiles[t][h] / tw[t];
                             }
                     bool ok = true;
                     for (int t = 0; t < 8; ++t)
-                        if (!isvalid(g, tiles[t])) ok = false;
+                        if (!isvalid64<EQ>(P, tiles[t])) ok = false;
                     if (ok) continue;
-                    Real means[5], twsum = Real(0);
+                    Real means[V], twsum = Real(0);
                     for (int t = 0; t < 8; ++t) twsum += tw[t];
-                    for (int h = 0; h < 5; ++h) {
+                    for (int h = 0; h < V; ++h) {
                         Real s = Real(0);
                         for (int t = 0; t < 8; ++t) s += tw[t] * tiles[t][h];
                         means[h] = s / twsum;
@@ -492,16 +234,16 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
                         if (squeeze < Real(0)) squeeze = Real(0);
                         bool okt = true;
                         for (int t = 0; t < 8; ++t) {
-                            Real tt[5];
-                            for (int h = 0; h < 5; ++h) tt[h] = means[h] + squeeze * (tiles[t][h] - means[h]);
-                            if (!isvalid(g, tt)) okt = false;
+                            Real tt[V];
+                            for (int h = 0; h < V; ++h) tt[h] = means[h] + squeeze * (tiles[t][h] - means[h]);
+                            if (!isvalid64<EQ>(P, tt)) okt = false;
                         }
                         if (okt) break;
                     }
                     for (int k = 2 * r * nn; k < (2 * r + 2) * nn; ++k)
                         for (int j = 2 * p * nn; j < (2 * p + 2) * nn; ++j)
                             for (int i = 2 * q * nn; i < (2 * q + 2) * nn; ++i)
-                                for (int h = 0; h < 5; ++h) {
+                                for (int h = 0; h < V; ++h) {
                                     Real* x = &su[h][base + swz<M>(i + M * j + M * M * k)];
                                     *x = means[h] + squeeze * (*x - means[h]);
                                 }
@@ -516,9 +258,9 @@ __device__ bool repair_element(const Gas<Real>& g, const Real* w, Real* const* s
 // max_nodes lambda_d, the fluid speed maximum for GLM cleaning, reduced to
 // one partial (dt_min, lambda_max, v_max) per block.  Thread t first folds
 // the M nodes of x-line t, then one thread per element folds its lines.
-template <int M, int BE, int R>
-__device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __restrict__ delta, long long e0,
-                                          long long nelem, double (*s_u)[BE * M * M * M], double* s_wbase,
+template <int EQ, int M, int BE, int R>
+__device__ __forceinline__ void block_cfl(const Phys64& P, const double* __restrict__ delta, long long e0,
+                                          long long nelem, const double (*s_u)[BE * M * M * M], double* s_wbase,
                                           double* __restrict__ out) {
     static_assert(R >= BE * M * M, "work rows must hold one value per thread");
     double (*s_w)[R] = reinterpret_cast<double (*)[R]>(s_wbase);
@@ -529,10 +271,11 @@ __device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __
     if (e0 + el < nelem) {
         for (int i = 0; i < M; ++i) {
             const int n = el * M3 + swz<M>(i + M * line);
-            double uu[5];
+            double uu[NVars<EQ>::V];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][n];
-            const double a0 = lambda_max<0>(g, uu), a1 = lambda_max<1>(g, uu), a2 = lambda_max<2>(g, uu);
+            for (int h = 0; h < NVars<EQ>::V; ++h) uu[h] = s_u[h][n];
+            const double a0 = lambda_max64<EQ, 0>(P, uu), a1 = lambda_max64<EQ, 1>(P, uu),
+                         a2 = lambda_max64<EQ, 2>(P, uu);
             l0 = a0 > l0 ? a0 : l0;
             l1 = a1 > l1 ? a1 : l1;
             l2 = a2 > l2 ? a2 : l2;
@@ -585,16 +328,26 @@ __device__ __forceinline__ void block_cfl(const Gas<double>& g, const double* __
     }
 }
 
-template <int M, int BE, int VOL, int SURF, typename Real>
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real>
 __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a) {
     using Sh = StageShape<M, BE>;
     constexpr int M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
+    static_assert(NT <= 64, "a workgroup is one wavefront");
+    constexpr int V = NVars<EQ>::V;
     constexpr bool F32 = sizeof(Real) == 4;
-    __shared__ double s_u[5][NN];          // FP64 state: RK update, repair, CFL
-    __shared__ Real s_acc[5][NN];          // RHS accumulators in operator precision
-    __shared__ double s_p64[F32 ? NN : 1]; // FP32 operator: FP64 pressure per node
-    __shared__ Real s_alpha[BE];
-    __shared__ int s_bad[BE];
+    // FP32 Euler operator: the operator reads an FP32 copy of the state (plus
+    // the FP64 pressure per node for the HLLC/HLLE blend) and the FP64 new
+    // state of the RK update is laid over the operator's LDS afterwards, so a
+    // workgroup needs 12 KB instead of 17.5 KB and more of them fit on a CU.
+    constexpr bool COMPACT = F32 && EQ == EQ_EULER;
+    using TU = typename std::conditional<COMPACT, float, double>::type;
+    constexpr int OP_DOUBLES = (V * NN * int(sizeof(TU)) + V * NN * int(sizeof(Real)) + 7) / 8;
+    constexpr int RAW_DOUBLES = OP_DOUBLES > V * NN ? OP_DOUBLES : V * NN;
+    __shared__ double s_raw[RAW_DOUBLES];
+    __shared__ double s_p64[COMPACT ? NN : 1];
+    TU (*s_u)[NN] = reinterpret_cast<TU (*)[NN]>(&s_raw[0]);
+    Real (*s_acc)[NN] = reinterpret_cast<Real (*)[NN]>(reinterpret_cast<TU*>(&s_raw[0]) + V * NN);
+    double (*s_u64)[NN] = reinterpret_cast<double (*)[NN]>(&s_raw[0]);  // FP64 state (RK update, repair, CFL)
 
     const int tid = threadIdx.x;
     const long long nblocks = (a.nelem + BE - 1) / BE;
@@ -602,45 +355,70 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     const long long e0 = blk * BE;
     const long long EM3 = a.nelem * (long long)M3;
     const long long nvalid_nodes = (a.nelem - e0 < BE ? a.nelem - e0 : BE) * (long long)M3;
+    // GLM cleaning speed and damping of this step (equations_share_mod)
+    double glm_alpha = 0.0;
+    MhdGas<Real> mg = a.mhd;
+    if constexpr (EQ == EQ_MHD) {
+        mg.ch = Real(a.glm[0]);
+        glm_alpha = a.glm[1];
+    }
 
     // ---- stage the block's state in LDS (coalesced) --------------------
-#pragma unroll
-    for (int h = 0; h < 5; ++h)
-        for (int idx = tid; idx < NN; idx += NT)
-            s_u[h][slot<M>(idx)] =
-                idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
-    if (tid < BE) {
-        s_alpha[tid] = Real(0);
-        s_bad[tid] = 0;
-    }
-    __syncthreads();
-    if constexpr (F32) {
+    if constexpr (COMPACT) {
         for (int idx = tid; idx < NN; idx += NT) {
             const int sl = slot<M>(idx);
             double uu[5];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][sl];
-            s_p64[sl] = pressure(a.gas64, uu);
+            for (int h = 0; h < 5; ++h) {
+                uu[h] = idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
+                s_u[h][sl] = float(uu[h]);
+            }
+            s_p64[sl] = pressure(a.ph64.gas, uu);
         }
-        __syncthreads();
+    } else {
+#pragma unroll
+        for (int h = 0; h < V; ++h)
+            for (int idx = tid; idx < NN; idx += NT)
+                s_u[h][slot<M>(idx)] =
+                    idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
     }
+    __syncthreads();
 
-    if (a.indicator == IND_PP) indicator_pp<M, BE, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
+    const Real alpha =
+        a.indicator == IND_PP ? indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64) : Real(0);
 
     // ---- three direction passes ----------------------------------------
-    direction_pass<0, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
+    if constexpr (EQ == EQ_MHD) {
+        direction_pass_mhd<0, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);
+        __syncthreads();
+        direction_pass_mhd<1, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);
+        __syncthreads();
+        direction_pass_mhd<2, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);
+    } else {
+        direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+        __syncthreads();
+        direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+        __syncthreads();
+        direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+    }
     __syncthreads();
-    direction_pass<1, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
-    __syncthreads();
-    direction_pass<2, M, BE, VOL, SURF, Real>(a, e0, s_u, s_acc, s_alpha, s_p64);
-    __syncthreads();
+
+    // Time derivative of variable h at block node slot sl, in FP64; GLM
+    // damping -alpha psi (kernel/mcfv/3d/kernel_flux_mod.f90:496-497) is part
+    // of the FP64 divergence-cleaning update.
+    auto dudt = [&](int h, int sl) {
+        double r = double(s_acc[h][sl]);
+        if constexpr (EQ == EQ_MHD)
+            if (h == GLMP) r = r - glm_alpha * double(s_u[GLMP][sl]);
+        return r;
+    };
 
     // ---- RHS only (parity path) -----------------------------------------
     if (a.rhs_out) {
 #pragma unroll
-        for (int h = 0; h < 5; ++h)
+        for (int h = 0; h < V; ++h)
             for (int idx = tid; idx < nvalid_nodes; idx += NT)
-                a.rhs_out[h * EM3 + e0 * M3 + idx] = double(s_acc[h][slot<M>(idx)]);
+                a.rhs_out[h * EM3 + e0 * M3 + idx] = dudt(h, slot<M>(idx));
         return;
     }
 
@@ -648,57 +426,91 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     //      or SSPRK(5,4) (ssp/4-5/timedisc_probe_mod.f90:24-46) ---------------
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     const int s = a.stage;
+    auto probe = [&](long long gi, double u, double rhs) {
+        double st;
+        if (a.timedisc == TD_LSRK54) {
+            const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * a.r0[gi] + rhs;
+            st = u + a.rk.lsrk_c[s - 1] * dt * reg;
+            a.r0[gi] = reg;
+        } else {
+            const double* k = a.rk.ssp;
+            switch (s) {
+            case 1: st = u + k[3] * dt * rhs; a.r0[gi] = u; break;
+            case 2: st = k[5] * a.r0[gi] + k[6] * u + k[7] * dt * rhs; break;
+            case 3: st = k[9] * a.r0[gi] + k[10] * u + k[11] * dt * rhs; break;
+            case 4:
+                st = k[13] * a.r0[gi] + k[14] * u + k[15] * dt * rhs;
+                a.r1[gi] = a.r1[gi] + k[22] * dt * rhs;
+                break;
+            default: st = k[17] * a.r0[gi] + k[18] * u + k[19] * dt * rhs + a.r1[gi]; break;
+            }
+        }
+        return st;
+    };
+    if constexpr (COMPACT) {
+        // FP64 state from HBM, new state in registers until the operator's
+        // LDS is free (each thread owns M node slots per variable)
+        double st[V][M];
 #pragma unroll
-    for (int h = 0; h < 5; ++h)
-        for (int idx = tid; idx < nvalid_nodes; idx += NT) {
-            const long long gi = h * EM3 + e0 * M3 + idx;
-            const int sl = slot<M>(idx);
-            const double rhs = double(s_acc[h][sl]);
-            const double u = s_u[h][sl];
-            double st;
-            if (a.timedisc == TD_LSRK54) {
-                const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * a.r0[gi] + rhs;
-                st = u + a.rk.lsrk_c[s - 1] * dt * reg;
-                a.r0[gi] = reg;
-            } else {
-                const double* k = a.rk.ssp;
-                switch (s) {
-                case 1: st = u + k[3] * dt * rhs; a.r0[gi] = u; break;
-                case 2: st = k[5] * a.r0[gi] + k[6] * u + k[7] * dt * rhs; break;
-                case 3: st = k[9] * a.r0[gi] + k[10] * u + k[11] * dt * rhs; break;
-                case 4:
-                    st = k[13] * a.r0[gi] + k[14] * u + k[15] * dt * rhs;
-                    a.r1[gi] = a.r1[gi] + k[22] * dt * rhs;
-                    break;
-                default: st = k[17] * a.r0[gi] + k[18] * u + k[19] * dt * rhs + a.r1[gi]; break;
+        for (int h = 0; h < V; ++h)
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                const int idx = tid + q * NT;
+                st[h][q] = 0.0;
+                if (idx < nvalid_nodes) {
+                    const long long gi = h * EM3 + e0 * M3 + idx;
+                    st[h][q] = probe(gi, a.u_in[gi], dudt(h, slot<M>(idx)));
                 }
             }
-            s_u[h][sl] = st;
-        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < V; ++h)
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                const int idx = tid + q * NT;
+                if (idx < nvalid_nodes) s_u64[h][slot<M>(idx)] = st[h][q];
+            }
+    } else {
+#pragma unroll
+        for (int h = 0; h < V; ++h)
+            for (int idx = tid; idx < nvalid_nodes; idx += NT) {
+                const long long gi = h * EM3 + e0 * M3 + idx;
+                const int sl = slot<M>(idx);
+                s_u64[h][sl] = probe(gi, s_u64[h][sl], dudt(h, sl));
+            }
+    }
     __syncthreads();
 
     // ---- positivity repair (kernel_repair after the probe) ---------------
     if (a.repair) {
+        unsigned bad = 0;
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
-            double uu[5];
+            double uu[V];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) uu[h] = s_u[h][slot<M>(idx)];
-            if (!isvalid(a.gas64, uu)) s_bad[idx / M3] = 1;
+            for (int h = 0; h < V; ++h) uu[h] = s_u64[h][slot<M>(idx)];
+            if (!isvalid64<EQ>(a.ph64, uu)) bad |= 1u << (idx / M3);
         }
-        __syncthreads();
-        if (tid < BE && s_bad[tid]) {
-            double* su[5] = {s_u[0], s_u[1], s_u[2], s_u[3], s_u[4]};
-            if (!repair_element<M, double>(a.gas64, a.rk.w64, su, tid * M3)) atomicOr(a.status, 1);
+        unsigned bad_el = 0;
+#pragma unroll
+        for (int el = 0; el < BE; ++el)
+            if (__any((bad >> el) & 1u)) bad_el |= 1u << el;
+        if (bad_el) {
+            if (tid < BE && ((bad_el >> tid) & 1u)) {
+                double* su[V];
+#pragma unroll
+                for (int h = 0; h < V; ++h) su[h] = s_u64[h];
+                if (!repair_element<EQ, M>(a.ph64, a.rk.w64, su, tid * M3)) atomicOr(a.status, 1);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
     // ---- write the new state; SSP register r1 takes the repaired state ---
 #pragma unroll
-    for (int h = 0; h < 5; ++h)
+    for (int h = 0; h < V; ++h)
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             const long long gi = h * EM3 + e0 * M3 + idx;
-            const double st = s_u[h][slot<M>(idx)];
+            const double st = s_u64[h][slot<M>(idx)];
             a.u_out[gi] = st;
             if (a.timedisc == TD_SSPRK54) {
                 if (s == 2) a.r1[gi] = a.rk.ssp[20] * st;
@@ -707,18 +519,16 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
         }
 
     // ---- CFL partials of the new state (last stage of a step) -----------
-    if (a.cfl_part)
-        block_cfl<M, BE, F32 ? NN / 2 : NN>(a.gas64, a.delta, e0, a.nelem, s_u, reinterpret_cast<double*>(&s_acc[0][0]),
-                                          a.cfl_part + 4 * blk);
+    if (a.cfl_part) block_cfl<EQ, M, BE, NT>(a.ph64, a.delta, e0, a.nelem, s_u64, &s_raw[0], a.cfl_part + 4 * blk);
 }
 
 // Standalone CFL reduction of a state in HBM (first step, calc_timestep):
 // the same per-block partials as the fused stage epilogue.
-template <int M, int BE>
+template <int EQ, int M, int BE>
 __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restrict__ u, const double* __restrict__ delta,
-                                                         long long nelem, Gas<double> g, double* __restrict__ part) {
-    constexpr int M3 = M * M * M, NN = BE * M3, NT = BE * M * M;
-    __shared__ double s_u[5][NN];
+                                                         long long nelem, Phys64 P, double* __restrict__ part) {
+    constexpr int M3 = M * M * M, NN = BE * M3, NT = BE * M * M, V = NVars<EQ>::V;
+    __shared__ double s_u[V][NN];
     __shared__ double s_w[4][NT];
     const int tid = threadIdx.x;
     const long long blk = blockIdx.x;
@@ -726,16 +536,19 @@ __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restric
     const long long EM3 = nelem * (long long)M3;
     const long long nvalid = (nelem - e0 < BE ? nelem - e0 : BE) * (long long)M3;
 #pragma unroll
-    for (int h = 0; h < 5; ++h)
+    for (int h = 0; h < V; ++h)
         for (int idx = tid; idx < NN; idx += NT)
             s_u[h][slot<M>(idx)] = idx < nvalid ? u[h * EM3 + e0 * M3 + idx] : (h == 0 || h == 4 ? 1.0 : 0.0);
     __syncthreads();
-    block_cfl<M, BE, NT>(g, delta, e0, nelem, s_u, &s_w[0][0], part + 4 * blk);
+    block_cfl<EQ, M, BE, NT>(P, delta, e0, nelem, s_u, &s_w[0][0], part + 4 * blk);
 }
 
 // Deterministic reduction of the per-block partials (min and max are exact,
 // so the result does not depend on the order) and
-// dt = 0.5*CFL/(2*N_NODES-1) * dtmin (kernel_timestep_mod.f90:49).
+// dt = 0.5*CFL/(2*N_NODES-1) * dtmin (kernel_timestep_mod.f90:49); for
+// GLM-MHD also c_h = 2|lmax - vmax| and alpha = c_h/0.18
+// (kernel/mcfv/3d/kernel_timestep_mod.f90:53-56).
+// dt_dev: [0] dt, [1] lmax, [2] vmax, [3] c_h, [4] alpha.
 __global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ part, long long npart, double cfl,
                                                       int M, double* __restrict__ dt_dev, double* __restrict__ hist,
                                                       long long slot) {
@@ -765,6 +578,9 @@ __global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ 
         dt_dev[0] = d;
         dt_dev[1] = s[1][0];
         dt_dev[2] = s[2][0];
+        const double ch = 2.0 * fabs(s[1][0] - s[2][0]);
+        dt_dev[3] = ch;
+        dt_dev[4] = ch / 0.18;
         if (hist && slot >= 0) hist[slot] = d;
     }
 }

@@ -240,7 +240,7 @@ struct Solver {
     double* alpha = nullptr;   // [nelem]
     int32_t* nbr = nullptr;
     double* delta = nullptr;
-    double* dt_dev = nullptr;  // [3]: dt, lmax, vmax
+    double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
     double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;
     double* hist = nullptr;
@@ -288,6 +288,25 @@ nemo::Gas<Real> make_gas(double k) {
     return g;
 }
 
+template <typename Real>
+nemo::MhdGas<Real> make_mhd(double k, double mu0) {
+    nemo::MhdGas<Real> g;
+    g.kappa = Real(k);
+    g.kappa_m1 = Real(k - 1.0);
+    g.kappa_m2 = Real(k - 2.0);
+    g.smu0 = Real(1.0 / mu0);
+    g.half_inv_km1 = Real(0.5 / (k - 1.0));
+    g.ch = Real(0);
+    return g;
+}
+
+nemo::Phys64 make_phys64(const nemo_gpu_config& c) {
+    nemo::Phys64 p;
+    p.gas = make_gas<double>(c.gamma);
+    p.mhd = make_mhd<double>(c.gamma, c.mu0);
+    return p;
+}
+
 template <int M, typename Real>
 nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
                                    double* alpha_out, double* cfl_part) {
@@ -301,6 +320,7 @@ nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double
     a.nbr = s.nbr;
     a.delta = s.delta;
     a.dt_dev = dt_dev;
+    a.glm = s.dt_dev + 3;
     a.status = s.status;
     a.cfl_part = cfl_part;
     a.nelem = s.nelem;
@@ -314,20 +334,21 @@ nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double
     a.ind_threshold = 0.5 * std::pow(10.0, -1.8 * std::pow(double(M), 0.25));
     a.ind_s = std::log((1.0 - 0.0001) / 0.0001);
     a.gas = make_gas<Real>(s.cfg.gamma);
-    a.gas64 = make_gas<double>(s.cfg.gamma);
+    a.mhd = make_mhd<Real>(s.cfg.gamma, s.cfg.mu0);
+    a.ph64 = make_phys64(s.cfg);
     a.op = make_table<M, Real>(s.ops);
     a.rk = s.rk;
     for (int i = 0; i < M; ++i) a.rk.w64[i] = s.ops.w[i];
     return a;
 }
 
-template <int M, int VOL, int SURF, typename Real>
+template <int EQ, int M, int VOL, int SURF, typename Real>
 void launch_stage_t(Solver& s, const nemo::StageArgs<M, Real>& a) {
     constexpr int BE = Tile<M>::BE;
     using Sh = nemo::StageShape<M, BE>;
     const long long nblocks = (s.nelem + BE - 1) / BE;
-    hipLaunchKernelGGL((nemo::stage_kernel<M, BE, VOL, SURF, Real>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0, s.stream,
-                       a);
+    hipLaunchKernelGGL((nemo::stage_kernel<EQ, M, BE, VOL, SURF, Real>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0,
+                       s.stream, a);
 }
 
 template <int M, typename Real>
@@ -335,19 +356,25 @@ int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, doubl
                     double* cfl_part) {
     const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
-#define NEMO_CASE(V_, S_)                              \
-    if (vol == V_ && surf == S_) {                     \
-        launch_stage_t<M, V_, S_, Real>(s, a);         \
+#define NEMO_CASE(E_, V_, S_)                          \
+    if (s.cfg.equations == E_ && vol == V_ && surf == S_) { \
+        launch_stage_t<E_, M, V_, S_, Real>(s, a);     \
         return NEMO_OK;                                \
     }
-    NEMO_CASE(nemo::VOL_WEAK, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::VOL_WEAK, nemo::SURF_HLLC)
-    NEMO_CASE(nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::VOL_CENTRAL, nemo::SURF_HLLC)
-    NEMO_CASE(nemo::VOL_EC, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::VOL_EC, nemo::SURF_HLLC)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_WEAK, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_WEAK, nemo::SURF_HLLC)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_CENTRAL, nemo::SURF_HLLC)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_HLLC)
+    if constexpr (M == 4) {
+        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
+        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_HLLD)
+        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_RUSANOV)
+        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_HLLD)
+    }
 #undef NEMO_CASE
-    return fail(NEMO_ERR_ARG, "unsupported volume/surface combination");
+    return fail(NEMO_ERR_ARG, "unsupported equations/volume/surface/n_poly combination");
 }
 
 template <int M>
@@ -402,9 +429,13 @@ long long nblocks_for(const Solver& s) {
 template <int M>
 void launch_cfl_m(Solver& s) {
     constexpr int BE = Tile<M>::BE;
-    const nemo::Gas<double> g = make_gas<double>(s.cfg.gamma);
-    hipLaunchKernelGGL((nemo::cfl_kernel<M, BE>), dim3(unsigned(stage_blocks<M>(s))), dim3(BE * M * M), 0,
-                       s.stream, s.u[s.cur], s.delta, s.nelem, g, s.part);
+    const nemo::Phys64 P = make_phys64(s.cfg);
+    if (s.cfg.equations == NEMO_EQ_GLM_MHD)
+        hipLaunchKernelGGL((nemo::cfl_kernel<nemo::EQ_MHD, M, BE>), dim3(unsigned(stage_blocks<M>(s))),
+                           dim3(BE * M * M), 0, s.stream, s.u[s.cur], s.delta, s.nelem, P, s.part);
+    else
+        hipLaunchKernelGGL((nemo::cfl_kernel<nemo::EQ_EULER, M, BE>), dim3(unsigned(stage_blocks<M>(s))),
+                           dim3(BE * M * M), 0, s.stream, s.u[s.cur], s.delta, s.nelem, P, s.part);
 }
 
 int launch_finalize(Solver& s, long long hist_slot) {
@@ -490,9 +521,19 @@ int nemo_gpu_operators(int32_t n_poly, double* x, double* w, double* D, double* 
 int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
     if (!cfg || !handle) return fail(NEMO_ERR_ARG, "null argument");
     if (cfg->n_poly < 2 || cfg->n_poly > 4) return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
-    if (cfg->equations != NEMO_EQ_EULER) return fail(NEMO_ERR_ARG, "equations: only NEMO_EQ_EULER is built");
+    if (cfg->equations != NEMO_EQ_EULER && cfg->equations != NEMO_EQ_GLM_MHD)
+        return fail(NEMO_ERR_ARG, "invalid equations");
     if (cfg->volume < 0 || cfg->volume > 2) return fail(NEMO_ERR_ARG, "invalid volume flux");
-    if (cfg->surface < 0 || cfg->surface > 1) return fail(NEMO_ERR_ARG, "invalid surface flux");
+    if (cfg->equations == NEMO_EQ_EULER && (cfg->surface < 0 || cfg->surface > 1))
+        return fail(NEMO_ERR_ARG, "invalid surface flux for Euler (Rusanov or HLLC)");
+    if (cfg->equations == NEMO_EQ_GLM_MHD) {
+        if (cfg->n_poly != 3) return fail(NEMO_ERR_ARG, "GLM-MHD is built for n_poly = 3");
+        if (cfg->volume == NEMO_VOL_WEAK)
+            return fail(NEMO_ERR_ARG, "GLM-MHD needs flux differencing (NEMO_VOL_CENTRAL or NEMO_VOL_EC)");
+        if (cfg->surface != NEMO_SURF_RUSANOV && cfg->surface != NEMO_SURF_HLLD)
+            return fail(NEMO_ERR_ARG, "invalid surface flux for GLM-MHD (Rusanov or HLLD)");
+        if (!(cfg->mu0 > 0.0)) return fail(NEMO_ERR_ARG, "mu0 must be positive");
+    }
     if (cfg->indicator < 0 || cfg->indicator > 1) return fail(NEMO_ERR_ARG, "invalid indicator");
     if (cfg->timedisc < 0 || cfg->timedisc > 1) return fail(NEMO_ERR_ARG, "invalid timedisc");
     if (cfg->precision < 0 || cfg->precision > 1) return fail(NEMO_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 operator)");
@@ -501,7 +542,7 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
     s->cfg = *cfg;
     s->M = cfg->n_poly + 1;
     s->M3 = s->M * s->M * s->M;
-    s->V = 5;
+    s->V = cfg->equations == NEMO_EQ_GLM_MHD ? 9 : 5;
     build_host_ops(cfg->n_poly, s->ops);
     s->rk = make_rk();
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -509,13 +550,14 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         delete s;
         return fail(NEMO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    if (hipMalloc(&s->dt_dev, 4 * sizeof(double)) != hipSuccess ||
+    if (hipMalloc(&s->dt_dev, 8 * sizeof(double)) != hipSuccess ||
         hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
         (void)hipStreamDestroy(s->stream);
         delete s;
         return fail(NEMO_ERR_HIP, "hipMalloc failed");
     }
     (void)hipMemset(s->status, 0, sizeof(int));
+    (void)hipMemset(s->dt_dev, 0, 8 * sizeof(double));
     *handle = s;
     return NEMO_OK;
 }
@@ -631,6 +673,27 @@ int nemo_gpu_calc_timestep(void* handle, double* dt, double* lmax, double* vmax)
     if (dt) *dt = h[0];
     if (lmax) *lmax = h[1];
     if (vmax) *vmax = h[2];
+    return NEMO_OK;
+}
+
+int nemo_gpu_set_glm(void* handle, double ch, double alpha) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    if (!(ch >= 0.0) || !(alpha >= 0.0)) return fail(NEMO_ERR_ARG, "GLM speed and damping must be >= 0");
+    const double v[2] = {ch, alpha};
+    HIP_TRY(hipMemcpyAsync(s->dt_dev + 3, v, sizeof(v), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_get_glm(void* handle, double* ch, double* alpha) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    double v[2];
+    HIP_TRY(hipMemcpyAsync(v, s->dt_dev + 3, sizeof(v), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (ch) *ch = v[0];
+    if (alpha) *alpha = v[1];
     return NEMO_OK;
 }
 

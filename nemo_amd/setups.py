@@ -88,6 +88,37 @@ def kelvin_helmholtz(mesh: Mesh, N: int, gamma: float = 1.4, seed: int = 1234,
     return np.ascontiguousarray(prim2cons(rho, vx, vy, vz, p, gamma))
 
 
+def mhd_prim2cons(rho, v, p, B, psi, gamma, mu0=1.0):
+    """prim2cons of equations/mhd/equations_mod.f90:294-310; v and B are
+    [3, ...] arrays."""
+    E = (p / (gamma - 1.0) + 0.5 * rho * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
+         + 0.5 / mu0 * (B[0] ** 2 + B[1] ** 2 + B[2] ** 2 + psi ** 2))
+    return np.stack([rho, rho * v[0], rho * v[1], rho * v[2], E, B[0], B[1], B[2], psi], axis=1)
+
+
+def orszag_tang(mesh: Mesh, N: int, gamma: float = 5.0 / 3.0, seed: int = 1234, noise: float = 1.0e-3) -> np.ndarray:
+    """Three-dimensional Orszag-Tang-type vortex on the periodic unit cube:
+    rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x, 0.1 sin 2 pi z),
+    B = (-sin 2 pi y, sin 4 pi x, 0)/sqrt(4 pi), psi = 0 (mu0 = 1), each velocity
+    component with seeded uniform noise in [-noise, noise].  (The reference's
+    setups/mhd/3d/orszag-tang/setup_init_mod.f90:44-64 differs in its
+    velocity and field.)"""
+    xi, _ = operators.lgl(N)
+    M3 = (N + 1) ** 3
+    X = mesh.node_coords(xi)
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    tp = 2.0 * np.pi
+    rho = np.full_like(x, 25.0 / (36.0 * np.pi))
+    p = np.full_like(x, 5.0 / (12.0 * np.pi))
+    lex = mesh.lex_id()
+    v = np.stack([-np.sin(tp * y) + noise * (2.0 * uniform01(seed, lex, M3, 1) - 1.0),
+                  np.sin(tp * x) + noise * (2.0 * uniform01(seed, lex, M3, 2) - 1.0),
+                  0.1 * np.sin(tp * z) + noise * (2.0 * uniform01(seed, lex, M3, 3) - 1.0)])
+    s = 1.0 / np.sqrt(4.0 * np.pi)
+    B = np.stack([-np.sin(tp * y) * s, np.sin(2.0 * tp * x) * s, np.zeros_like(x)])
+    return np.ascontiguousarray(mhd_prim2cons(rho, v, p, B, np.zeros_like(x), gamma))
+
+
 def config_c1(n: int = 8, N: int = 3):
     """Config 1: 8^3 periodic unit cube, N=3, isentropic vortex, FP64."""
     mesh = uniform_periodic(n)
@@ -98,3 +129,9 @@ def config_c2(n: int = 64, N: int = 3, seed: int = 1234):
     """Config 2: 64^3 periodic unit cube, N=3, Kelvin-Helmholtz, FP64."""
     mesh = uniform_periodic(n)
     return mesh, kelvin_helmholtz(mesh, N, seed=seed)
+
+
+def config_c3(n: int = 48, N: int = 3, seed: int = 1234):
+    """Config 3: 48^3 periodic unit cube, N=3, GLM-MHD Orszag-Tang-type vortex."""
+    mesh = uniform_periodic(n)
+    return mesh, orszag_tang(mesh, N, seed=seed)

@@ -58,9 +58,9 @@ class Runtime:
 def make_config(N: int = 3, volume: int = _lib.VOL_WEAK, surface: int = _lib.SURF_RUSANOV,
                 indicator: int = _lib.IND_NONE, timedisc: int = _lib.TD_SSPRK54, repair: int = 0,
                 gamma: float = 1.4, alpha_max: float = 0.5, alpha_min: float = 0.001, cfl: float = 0.5,
-                precision: int = 0) -> Config:
-    return Config(N, _lib.EQ_EULER, volume, surface, indicator, timedisc, repair, precision, gamma,
-                  alpha_max, alpha_min, cfl, 1.0)
+                precision: int = 0, equations: int = _lib.EQ_EULER, mu0: float = 1.0) -> Config:
+    return Config(N, equations, volume, surface, indicator, timedisc, repair, precision, gamma,
+                  alpha_max, alpha_min, cfl, mu0)
 
 
 class Solver:
@@ -70,7 +70,7 @@ class Solver:
         self.cfg = cfg
         self.M = cfg.n_poly + 1
         self.M3 = self.M ** 3
-        self.nvars = 5
+        self.nvars = 9 if cfg.equations == _lib.EQ_GLM_MHD else 5
         self.runtime = Runtime()
         self._h = ctypes.c_void_p()
         self._call("nemo_gpu_create", ctypes.byref(cfg), ctypes.byref(self._h))
@@ -138,6 +138,15 @@ class Solver:
         self._call("nemo_gpu_calc_timestep", self._h, ctypes.byref(dt), ctypes.byref(lm), ctypes.byref(vm))
         self.runtime.maxtimestep = dt.value
         return dt.value, lm.value, vm.value
+
+    def set_glm(self, ch: float, alpha: float) -> None:
+        """equations_share_mod::glm_ch / glm_alpha (GLM-MHD)."""
+        self._call("nemo_gpu_set_glm", self._h, ctypes.c_double(ch), ctypes.c_double(alpha))
+
+    def get_glm(self):
+        ch, alpha = ctypes.c_double(), ctypes.c_double()
+        self._call("nemo_gpu_get_glm", self._h, ctypes.byref(ch), ctypes.byref(alpha))
+        return ch.value, alpha.value
 
     # -- timedisc interface ----------------------------------------------------
     def dt_coeffs(self):

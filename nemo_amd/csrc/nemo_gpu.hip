@@ -294,6 +294,7 @@ nemo::MhdGas<Real> make_mhd(double k, double mu0) {
     g.kappa = Real(k);
     g.kappa_m1 = Real(k - 1.0);
     g.kappa_m2 = Real(k - 2.0);
+    g.mu0 = Real(mu0);
     g.smu0 = Real(1.0 / mu0);
     g.half_inv_km1 = Real(0.5 / (k - 1.0));
     g.ch = Real(0);

@@ -34,17 +34,71 @@ struct Gas {
     Real half_inv_km1; // 0.5/(kappa-1)
 };
 
-// Arithmetic by operator precision.  FP64 keeps IEEE division, sqrt and log
-// (the plain path matches the reference to round-off); the FP32 operator
-// uses the hardware reciprocal, square root and log2 (v_rcp_f32, v_sqrt_f32,
-// v_log_f32, about 1 ulp), which is what makes it cheap.
-__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+// Arithmetic by operator precision.  Where the reference's result is not
+// sensitive to the last bit, divisions and reciprocals use the hardware
+// reciprocal estimate: v_rcp_f32 for FP32 (about 1 ulp), and for FP64
+// v_rcp_f64 refined by two Newton steps (about 1 ulp, 5 instructions against
+// 10 for an IEEE division).  IEEE division stays where bits matter: the
+// pressure (see pressure_ordered), the ratio a/b of the log-mean and the CFL
+// speeds.
+__device__ __forceinline__ double rcp_nr(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double fdiv(double a, double b) { return a * rcp_nr(b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double frcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ double frcp(double a) { return rcp_nr(a); }
 __device__ __forceinline__ float frcp(float a) { return __builtin_amdgcn_rcpf(a); }
-__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+// Square root to about 1 ulp for a >= 0: v_rsq_f64 estimate, one Goldschmidt
+// step and a Newton correction (8 instructions against about 15 for the
+// IEEE-rounded sqrt with its range scaling; arguments here are O(1e-12..1e12)).
+__device__ __forceinline__ double fsqrt(double a) {
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, a);
+    g = fma(d, h, g);
+    return a > 0.0 ? g : 0.0;
+}
 __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
-__device__ __forceinline__ double flog(double a) { return log(a); }
+
+// Natural logarithm of a positive finite x to about 2 ulp (the library log
+// carries double-double arithmetic for 0.5 ulp, three times the work):
+// x = m 2^k with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+// s = (m-1)/(m+1), |s| <= 0.1716, summed as 2s + 2s^3 sum_j s^2j/(2j+3) up to
+// s^23 (truncation below 1e-17 relative; Abramowitz & Stegun 4.1.27), plus
+// k log 2.  m - 1 is exact, so the result keeps its relative accuracy for x
+// near 1, where the log-mean divides by it.
+__device__ __forceinline__ double log_pos(double x) {
+    int k = __builtin_amdgcn_frexp_exp(x);
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m = m + m;
+        k = k - 1;
+    }
+    const double f = m - 1.0;
+    const double s = f * rcp_nr(m + 1.0);
+    const double t = s * s;
+    double P = 1.0 / 23.0;
+    P = fma(P, t, 1.0 / 21.0);
+    P = fma(P, t, 1.0 / 19.0);
+    P = fma(P, t, 1.0 / 17.0);
+    P = fma(P, t, 1.0 / 15.0);
+    P = fma(P, t, 1.0 / 13.0);
+    P = fma(P, t, 1.0 / 11.0);
+    P = fma(P, t, 1.0 / 9.0);
+    P = fma(P, t, 1.0 / 7.0);
+    P = fma(P, t, 1.0 / 5.0);
+    P = fma(P, t, 1.0 / 3.0);
+    const double lm = fma(2.0 * s * t, P, 2.0 * s);
+    return fma(double(k), 0.69314718055994530942, lm);
+}
+__device__ __forceinline__ double flog(double a) { return log_pos(a); }
 __device__ __forceinline__ float flog(float a) { return __builtin_amdgcn_logf(a) * 0.69314718055994530942f; }
 
 // pressure_elemental: equations/euler/equations_mod.f90:131-140.
@@ -188,7 +242,7 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
         flux_s<D>(R, F_R);
         Real smooth;
         if constexpr (sizeof(Real) == 8) {
-            smooth = sqrt(fabs(p_L - p_R) / (p_L + p_R));
+            smooth = fsqrt(fabs(p_L - p_R) * frcp(p_L + p_R));
         } else {
             // The blend weight sqrt(|dp|/sum p) turns a relative pressure
             // error e into a weight of about sqrt(e): FP32 pressures would
@@ -224,18 +278,17 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
 // 13-32 (series branch as written, including the sign of its u^2 term),
 // returned together with its reciprocal so that the flux needs no further
 // division by it: 1/L = log(a/b)/(a-b) on the log branch.
-template <typename Real>
-__device__ __forceinline__ void logmean2(Real a, Real b, Real& lm, Real& inv_lm) {
-    const Real x = a / b;
-    if (fabs(a - b) < Real(1e-4)) {
-        const Real c1 = Real(1.0 / 6.0), c2 = Real(2.0 / 45.0), c3 = Real(22.0 / 945.0);
-        const Real u = (x * (x - Real(2)) + Real(1)) / (x * (x + Real(2)) + Real(1));
-        lm = (a + b) * (Real(0.5) - u * (c1 - u * (c2 - c3 * u)));
-        inv_lm = Real(1) / lm;
+__device__ __forceinline__ void logmean2(double a, double b, double& lm, double& inv_lm) {
+    const double x = a / b;  // IEEE: log(x) amplifies its rounding by 1/|x-1|
+    if (fabs(a - b) < 1e-4) {
+        const double c1 = 1.0 / 6.0, c2 = 2.0 / 45.0, c3 = 22.0 / 945.0;
+        const double u = (x * (x - 2.0) + 1.0) * rcp_nr(x * (x + 2.0) + 1.0);
+        lm = (a + b) * (0.5 - u * (c1 - u * (c2 - c3 * u)));
+        inv_lm = rcp_nr(lm);
     } else {
-        const Real l = log(x);
-        lm = (a - b) / l;
-        inv_lm = l / (a - b);
+        const double l = log_pos(x);
+        lm = (a - b) * rcp_nr(l);
+        inv_lm = l * rcp_nr(a - b);
     }
 }
 

@@ -20,6 +20,7 @@ enum { MAGX = 5, MAGY = 6, MAGZ = 7, GLMP = 8 };
 template <typename Real>
 struct MhdGas {
     Real kappa, kappa_m1, kappa_m2;
+    Real mu0;
     Real smu0;         // 1/mu0
     Real half_inv_km1; // 0.5/(kappa-1)
     Real ch;           // GLM cleaning speed of the current step
@@ -32,11 +33,13 @@ struct Mag {
 };
 
 // pressure_elemental: equations/mhd/equations_mod.f90:223-234 (gas pressure,
-// psi^2/2 taken out of the energy as well).
+// psi^2/2 taken out of the energy as well), in the reference's order and
+// without FMA contraction: the CFL reduction is compared bit for bit.
 template <typename Real>
 __device__ __forceinline__ Real mhd_pressure(const MhdGas<Real>& g, const Real* u) {
+#pragma clang fp contract(off)
     return g.kappa_m1 * (u[ENER] - Real(0.5) / u[DENS] * (u[MOMX] * u[MOMX] + u[MOMY] * u[MOMY] + u[MOMZ] * u[MOMZ]) -
-                         Real(0.5) * g.smu0 * (u[MAGX] * u[MAGX] + u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ] + u[GLMP] * u[GLMP]));
+                         Real(0.5) / g.mu0 * (u[MAGX] * u[MAGX] + u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ] + u[GLMP] * u[GLMP]));
 }
 
 template <typename Real>
@@ -92,14 +95,26 @@ __device__ __forceinline__ void mhd_flux_s(const MhdGas<Real>& g, const MhdState
 }
 
 // Fastest_Signal_Speed (equations/mhd/equations_mod.f90:693-714) in direction D.
-template <int D, typename Real>
+// EXACT: IEEE square roots (the CFL reduction, compared bit for bit).
+template <int D, bool EXACT = false, typename Real>
 __device__ __forceinline__ Real mhd_fastspeed(const MhdGas<Real>& g, const MhdState<Real>& s) {
     constexpr int BN = Mag<D, 0>::idx, BT1 = Mag<D, 1>::idx, BT2 = Mag<D, 2>::idx;
-    const Real cs2 = g.kappa * s.p * s.inv_rho;
-    const Real ca2 = s.u[BN] * s.u[BN] * s.inv_rho * g.smu0;
-    const Real va2 = (s.u[BT1] * s.u[BT1] + s.u[BT2] * s.u[BT2]) * s.inv_rho * g.smu0 + ca2;
-    const Real astar = fsqrt((cs2 + va2) * (cs2 + va2) - Real(4) * cs2 * ca2);
-    return fsqrt(Real(0.5) * (cs2 + va2 + astar));
+    if constexpr (EXACT) {
+#pragma clang fp contract(off)
+        // the reference's expression order, IEEE divisions and roots
+        const Real rho = s.u[DENS], mu0 = g.mu0;
+        const Real cs2 = g.kappa * s.p / rho;
+        const Real ca2 = s.u[BN] * s.u[BN] / rho / mu0;
+        const Real va2 = (s.u[BT1] * s.u[BT1] + s.u[BT2] * s.u[BT2]) / rho / mu0 + ca2;
+        const Real astar = sqrt((cs2 + va2) * (cs2 + va2) - Real(4) * cs2 * ca2);
+        return sqrt(Real(0.5) * (cs2 + va2 + astar));
+    } else {
+        const Real cs2 = g.kappa * s.p * s.inv_rho;
+        const Real ca2 = s.u[BN] * s.u[BN] * s.inv_rho * g.smu0;
+        const Real va2 = (s.u[BT1] * s.u[BT1] + s.u[BT2] * s.u[BT2]) * s.inv_rho * g.smu0 + ca2;
+        const Real astar = fsqrt((cs2 + va2) * (cs2 + va2) - Real(4) * cs2 * ca2);
+        return fsqrt(Real(0.5) * (cs2 + va2 + astar));
+    }
 }
 
 // x/y/zlambdaMax (equations/mhd/equations_mod.f90:522-690) from the raw
@@ -114,7 +129,7 @@ __device__ __forceinline__ double mhd_lambda_max(const MhdGas<double>& g, const 
     s.v[1] = u[MOMY] / u[DENS];
     s.v[2] = u[MOMZ] / u[DENS];
     s.p = mhd_pressure(g, u);
-    return fabs(s.v[D]) + mhd_fastspeed<D>(g, s);
+    return fabs(s.v[D]) + mhd_fastspeed<D, true>(g, s);
 }
 
 // Rusanov: equations/mhd/riemann/rusanov/riemann_mod.f90:7-24
@@ -153,14 +168,15 @@ __device__ __forceinline__ void mhd_nc_add(const MhdGas<Real>& g, const MhdState
 // its reciprocal.  FP32: the Ismail-Roe series form of logmean2.
 __device__ __forceinline__ void ln_mean(double aL, double aR, double& lm, double& inv_lm) {
     const double Xi = aR / aL;
-    const double u = (Xi * (Xi - 2.0) + 1.0) / (Xi * (Xi + 2.0) + 1.0);
+    const double u = (Xi * (Xi - 2.0) + 1.0) * rcp_nr(Xi * (Xi + 2.0) + 1.0);
     if (u < 1.0e-4) {
-        lm = (aL + aR) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + u * 15.0)));
-        inv_lm = (105.0 + u * (35.0 + u * (21.0 + u * 15.0))) / ((aL + aR) * 52.5);
+        const double d = 105.0 + u * (35.0 + u * (21.0 + u * 15.0));
+        lm = (aL + aR) * 52.5 * rcp_nr(d);
+        inv_lm = d * rcp_nr((aL + aR) * 52.5);
     } else {
-        const double l = log(Xi);
-        lm = (aR - aL) / l;
-        inv_lm = l / (aR - aL);
+        const double l = log_pos(Xi);
+        lm = (aR - aL) * rcp_nr(l);
+        inv_lm = l * rcp_nr(aR - aL);
     }
 }
 __device__ __forceinline__ void ln_mean(float aL, float aR, float& lm, float& inv_lm) { logmean2(aL, aR, lm, inv_lm); }

@@ -53,6 +53,8 @@ extern "C" {
 /* troubled-cell indicator for DG/FV blending */
 #define NEMO_IND_NONE 0        /* pure DG */
 #define NEMO_IND_PP 1          /* Persson-Peraire on rho*p */
+#define NEMO_IND_MLP 2         /* learned MLP indicator, Persson-Peraire as judge and fallback
+                                  (Euler, n_poly = 3, EC volume flux, HLLC or Rusanov) */
 /* time integration (timedisc/rungekutta/...) */
 #define NEMO_TD_LSRK54 0       /* low-storage/4-5 */
 #define NEMO_TD_SSPRK54 1      /* ssp/4-5 */
@@ -142,6 +144,39 @@ int nemo_gpu_get_status(void *handle, int32_t *code);
  * (reading synchronises and resets the counters). */
 int nemo_gpu_set_timing(void *handle, int32_t enable);
 int nemo_gpu_read_timing(void *handle, int64_t *launches, double *total_ms);
+
+/* ---- learned troubled-cell indicator --------------------------------------
+ * No counterpart in the reference, whose DG/FV blending survives only as
+ * traces (setups/mhd/3d/_common_/setup_checkpoint_mod.f90:395-450).
+ * Network: 4 inputs -> 3 hidden layers of 64 (GELU) -> sigmoid alpha.
+ * Inputs per element: 0.1 ln(f + 1e-12) for the modal energy fractions
+ * f = E(top shell)/E, E(top two shells)/E of rho and of p.
+ * Parameters (FP32, NEMO_MLP_NPARAMS): W1[64][4], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], w4[64], b4, W[j][k] from input k to unit j.
+ * With NEMO_IND_MLP the factor is computed at the first stage of every time
+ * step and kept for the step; the Persson-Peraire indicator judges a sampled
+ * fraction of the elements each step and takes over where the prediction is
+ * short by more than the tolerance, and where rho or p is non-positive. */
+#define NEMO_MLP_NPARAMS 8705
+int nemo_gpu_mlp_set_params(void *handle, const float *params);
+int nemo_gpu_mlp_get_params(void *handle, float *params);
+/* judge: sampled fraction of elements per step (default 0.05) and tolerance (0.05) */
+int nemo_gpu_mlp_set_policy(void *handle, double sample_fraction, double tolerance);
+/* judged, overridden and fallback element counts since the last call */
+int nemo_gpu_mlp_stats(void *handle, int64_t *judged, int64_t *overridden, int64_t *fallback);
+/* Training data of the current state: features[nelem][4] and the
+ * Persson-Peraire factor alpha_pp[nelem] (any Euler scheme, n_poly = 3). */
+int nemo_gpu_indicator_features(void *handle, float *features, double *alpha_pp);
+/* Network output for n feature rows. */
+int nemo_gpu_mlp_forward(void *handle, int64_t n, const float *features, float *y);
+/* Loss 1/(2n) sum (y - target)^2 and its gradient (canonical layout). */
+int nemo_gpu_mlp_grad(void *handle, int64_t n, const float *features, const float *target, float *loss,
+                      float *grad);
+/* Adam training on the device from the current parameters: `epochs` passes
+ * over shuffled minibatches of `batch` rows; loss_per_epoch (may be NULL)
+ * receives the mean minibatch loss of each epoch. */
+int nemo_gpu_mlp_train(void *handle, int64_t n, const float *features, const float *target, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
 
 /* Operators for polynomial degree N (host only, no device needed):
  * LGL nodes x[M], weights w[M] (utils/interpol_mod.f90:513-582),

@@ -18,7 +18,8 @@ NEMO_ERR_NOT_READY = 4
 EQ_EULER, EQ_GLM_MHD = 0, 1
 VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
 SURF_RUSANOV, SURF_HLLC, SURF_HLLD = 0, 1, 2
-IND_NONE, IND_PP = 0, 1
+IND_NONE, IND_PP, IND_MLP = 0, 1, 2
+MLP_NPARAMS = 8705
 TD_LSRK54, TD_SSPRK54 = 0, 1
 
 
@@ -50,6 +51,7 @@ class NemoGpuError(RuntimeError):
 _lib = None
 
 _P = ctypes.POINTER(ctypes.c_double)
+_PF = ctypes.POINTER(ctypes.c_float)
 _PI32 = ctypes.POINTER(ctypes.c_int32)
 _PI64 = ctypes.POINTER(ctypes.c_int64)
 _VP = ctypes.c_void_p
@@ -70,6 +72,15 @@ SIGNATURES = {
     "nemo_gpu_run": [_VP, ctypes.c_int32, _P],
     "nemo_gpu_synchronize": [_VP],
     "nemo_gpu_get_status": [_VP, _PI32],
+    "nemo_gpu_indicator_features": [_VP, _PF, _P],
+    "nemo_gpu_mlp_set_params": [_VP, _PF],
+    "nemo_gpu_mlp_get_params": [_VP, _PF],
+    "nemo_gpu_mlp_set_policy": [_VP, ctypes.c_double, ctypes.c_double],
+    "nemo_gpu_mlp_stats": [_VP, _PI64, _PI64, _PI64],
+    "nemo_gpu_mlp_forward": [_VP, ctypes.c_int64, _PF, _PF],
+    "nemo_gpu_mlp_grad": [_VP, ctypes.c_int64, _PF, _PF, _PF, _PF],
+    "nemo_gpu_mlp_train": [_VP, ctypes.c_int64, _PF, _PF, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                           ctypes.c_uint64, _PF],
     "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
     "nemo_gpu_read_timing": [_VP, _PI64, _P],
     "nemo_gpu_operators": [ctypes.c_int32, _P, _P, _P, _P],
@@ -106,6 +117,10 @@ def check(rc: int, where: str) -> None:
 
 def dptr(a):
     return a.ctypes.data_as(_P)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_PF)
 
 
 def iptr(a):

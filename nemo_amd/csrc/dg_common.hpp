@@ -14,7 +14,7 @@ enum { EQ_EULER = 0, EQ_MHD = 1 };
 enum { VOL_WEAK = 0, VOL_CENTRAL = 1, VOL_EC = 2 };
 enum { SURF_RUSANOV = 0, SURF_HLLC = 1, SURF_HLLD = 2 };
 enum { TD_LSRK54 = 0, TD_SSPRK54 = 1 };
-enum { IND_NONE = 0, IND_PP = 1 };
+enum { IND_NONE = 0, IND_PP = 1, IND_MLP = 2 };
 
 // Operator tables for M LGL nodes (built on the host, passed by value).
 template <int M, typename Real>
@@ -80,6 +80,15 @@ struct StageArgs {
     const double* __restrict__ glm;  // MHD: [c_h, alpha] of the current step
     int* __restrict__ status;
     double* __restrict__ cfl_part;   // non-null: per-block CFL partials of the new state
+    // learned indicator
+    const float* __restrict__ mlp;   // network parameters, inference layout
+    double* __restrict__ alpha_step; // per element factor of the current step
+    float* __restrict__ feat_out;    // non-null: write features [nelem][4] (training data)
+    unsigned long long* __restrict__ mlp_stats;  // judged, overridden, fallback
+    int alpha_refresh;               // 1: compute the factor (first stage of a step)
+    uint32_t step_index;             // sampling key of the judge
+    uint32_t sample_thresh;          // judged fraction * 2^32
+    double judge_tol;                // PP takes over where it exceeds the prediction by this
     long long nelem;
     double dt;
     int stage;      // 1..5

@@ -31,39 +31,24 @@
 #include "dg_common.hpp"
 #include "dg_euler.hpp"
 #include "dg_mhd.hpp"
+#include "mlp.hpp"
 
 namespace nemo {
 
-// Persson-Peraire indicator on rho*p (Hennemann et al. 2021, sec. 3): modal
-// energies of the normalised-Legendre expansion (utils/interpol_mod.f90:
-// 274-311, 584-601), alpha from the logistic ramp around the threshold
-// 0.5*10^(-1.8*M^0.25), clipped by alpha_min/alpha_max.
-//
-// Returns the blending factor of the calling thread's element (one
-// workgroup is one wavefront, so it is broadcast by a lane shuffle).
-template <int EQ, int M, int BE, typename Real, typename TU>
-__device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long long e0,
-                                             const TU (*s_u)[BE * M * M * M], Real (*s_acc)[BE * M * M * M],
-                                             const double* s_p64) {
-    constexpr int V = NVars<EQ>::V;
-    using Sh = StageShape<M, BE>;
-    constexpr int M2 = Sh::M2, M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
+// Modal energies of a nodal field q held in LDS (block slot order) in the
+// normalised-Legendre basis (utils/interpol_mod.f90:274-311, 584-601): the
+// sum-factorised transform x -> y -> z, after which the calling thread holds
+// its line's partial sums of the total energy (tot), of the modes with all
+// indices below M-1 (c1) and of those with all indices below M-2 (c2), and
+// the energies of the top shell (some index M-1: s1 = tot - c1) and of the
+// top two shells (some index >= M-2: s2 = tot - c2) summed directly, free
+// of the cancellation in tot - c.
+// q must be complete (barrier) on entry; t1 and t2 are work rows.
+template <int M, int BE, typename Real>
+__device__ __forceinline__ void modal_partials(const StageArgs<M, Real>& a, const Real* q, Real* t1, Real* t2,
+                                               Real& tot, Real& c1, Real& c2, Real& s1, Real& s2) {
+    constexpr int M2 = M * M, M3 = M2 * M;
     const int tid = threadIdx.x;
-    Real* q = s_acc[2];
-    Real* t1 = s_acc[0];
-    Real* t2 = s_acc[1];
-    for (int idx = tid; idx < NN; idx += NT) {
-        const int sl = slot<M>(idx);
-        if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
-            q[sl] = Real(double(s_u[DENS][sl]) * s_p64[sl]);
-        } else {
-            double uu[V];
-#pragma unroll
-            for (int h = 0; h < V; ++h) uu[h] = s_u[h][sl];
-            q[sl] = Real(uu[DENS] * pressure64<EQ>(a.ph64, uu));
-        }
-    }
-    __syncthreads();
     const int el = tid / M2, line = tid - el * M2;
     const int ta = line % M, tb = line / M;
     const int base = el * M3;
@@ -97,7 +82,7 @@ __device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long l
         Real v[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) v[k] = t2[base + swz<M>(ta + M * tb + M2 * k)];
-        Real tot = Real(0), c1 = Real(0), c2 = Real(0);
+        tot = c1 = c2 = s1 = s2 = Real(0);
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
@@ -105,42 +90,229 @@ __device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long l
             for (int k = 0; k < M; ++k) s += a.op.Vinv[r][k] * v[k];
             const Real e2 = s * s;
             tot += e2;
-            if (ta < M - 1 && tb < M - 1 && r < M - 1) c1 += e2;
-            if (ta < M - 2 && tb < M - 2 && r < M - 2) c2 += e2;
+            if (ta < M - 1 && tb < M - 1 && r < M - 1)
+                c1 += e2;
+            else
+                s1 += e2;
+            if (ta < M - 2 && tb < M - 2 && r < M - 2)
+                c2 += e2;
+            else
+                s2 += e2;
         }
-        // q (row 2) is dead after the x contraction; rows 2..4 take the partials
-        s_acc[2][tid] = tot;
-        s_acc[3][tid] = c1;
-        s_acc[4][tid] = c2;
+    }
+}
+
+// Blending factor from the Persson-Peraire energy max((tot-c1)/tot,
+// (tot-c2)/tot): logistic ramp with sharpness s = log(9999) around the
+// threshold 0.5*10^(-1.8*M^0.25), values below alpha_min set to 0 (above
+// 1-alpha_min to 1), clipped at alpha_max.
+template <int M, typename Real>
+__device__ __forceinline__ double pp_alpha(const StageArgs<M, Real>& a, Real tot, Real c1, Real c2) {
+    double energy = 0.0;
+    if (tot > Real(0)) {
+        const double f1 = double(tot - c1) / double(tot);
+        const double f2 = double(tot - c2) / double(tot);
+        energy = f1 > f2 ? f1 : f2;
+    }
+    double alpha = 1.0 / (1.0 + exp(-a.ind_s / a.ind_threshold * (energy - a.ind_threshold)));
+    if (alpha < a.alpha_min)
+        alpha = 0.0;
+    else if (alpha > 1.0 - a.alpha_min)
+        alpha = 1.0;
+    if (alpha > a.alpha_max) alpha = a.alpha_max;
+    return alpha;
+}
+
+// Persson-Peraire indicator on rho*p (Hennemann et al. 2021, sec. 3).
+// Returns the blending factor of the calling thread's element (one
+// workgroup is one wavefront, so it is broadcast by a lane shuffle); with
+// write_out the factor also goes to alpha_out.
+template <int EQ, int M, int BE, typename Real, typename TU>
+__device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long long e0,
+                                             const TU (*s_u)[BE * M * M * M], Real (*s_acc)[BE * M * M * M],
+                                             const double* s_p64, bool write_out = true) {
+    constexpr int V = NVars<EQ>::V;
+    using Sh = StageShape<M, BE>;
+    constexpr int M2 = Sh::M2, NN = Sh::NN, NT = Sh::NT;
+    const int tid = threadIdx.x;
+    Real* q = s_acc[2];
+    for (int idx = tid; idx < NN; idx += NT) {
+        const int sl = slot<M>(idx);
+        if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
+            q[sl] = Real(double(s_u[DENS][sl]) * s_p64[sl]);
+        } else {
+            double uu[V];
+#pragma unroll
+            for (int h = 0; h < V; ++h) uu[h] = s_u[h][sl];
+            q[sl] = Real(uu[DENS] * pressure64<EQ>(a.ph64, uu));
+        }
     }
     __syncthreads();
+    Real tot, c1, c2, s1, s2;
+    modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, c2, s1, s2);
+    // q (row 2) is dead after the x contraction; rows 2..4 take the partials
+    s_acc[2][tid] = tot;
+    s_acc[3][tid] = c1;
+    s_acc[4][tid] = c2;
+    __syncthreads();
+    const int el = tid / M2, line = tid - el * M2;
     Real alpha_el = Real(0);
     if (line == 0) {
-        Real tot = Real(0), c1 = Real(0), c2 = Real(0);
+        Real st = Real(0), s1 = Real(0), s2 = Real(0);
         for (int l = 0; l < M2; ++l) {
-            tot += s_acc[2][el * M2 + l];
-            c1 += s_acc[3][el * M2 + l];
-            c2 += s_acc[4][el * M2 + l];
+            st += s_acc[2][el * M2 + l];
+            s1 += s_acc[3][el * M2 + l];
+            s2 += s_acc[4][el * M2 + l];
         }
-        double energy = 0.0;
-        if (tot > Real(0)) {
-            const double f1 = double(tot - c1) / double(tot);
-            const double f2 = double(tot - c2) / double(tot);
-            energy = f1 > f2 ? f1 : f2;
-        }
-        double alpha = 1.0 / (1.0 + exp(-a.ind_s / a.ind_threshold * (energy - a.ind_threshold)));
-        if (alpha < a.alpha_min)
-            alpha = 0.0;
-        else if (alpha > 1.0 - a.alpha_min)
-            alpha = 1.0;
-        if (alpha > a.alpha_max) alpha = a.alpha_max;
+        const double alpha = pp_alpha(a, st, s1, s2);
         alpha_el = Real(alpha);
         const long long e = e0 + el;
-        if (a.alpha_out && e < a.nelem) a.alpha_out[e] = alpha;
+        if (write_out && a.alpha_out && e < a.nelem) a.alpha_out[e] = alpha;
     }
     alpha_el = __shfl(alpha_el, el * M2);
     __syncthreads();  // the work rows are the direction passes' accumulators
     return alpha_el;
+}
+
+// Element sums of the line partials over the 16 lanes of an element (M = 4).
+template <typename Real>
+__device__ __forceinline__ Real sum16(Real v) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Counter-based sampling key of (element, step).
+__device__ __forceinline__ uint32_t sample_hash(long long e, uint32_t step) {
+    uint32_t h = uint32_t(e) * 0x9E3779B1u ^ (uint32_t(e >> 32) * 0x85EBCA77u) ^ (step * 0xC2B2AE3Du);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
+// Learned indicator (M = 4, one element per 16 lanes).  At the first stage
+// of a step (alpha_refresh) it computes per element the features
+//   0.1 ln(f + 1e-12),  f = E(top shell)/E, E(top two shells)/E of rho and p,
+// evaluates the network, and lets the Persson-Peraire indicator judge a
+// sampled fraction of the elements (sample_hash < sample_thresh): where the
+// prediction falls short of the PP factor by more than judge_tol, or where
+// any node has rho <= 0 or p <= 0, the PP factor is used.  The factor is
+// kept in alpha_step for the remaining stages of the step.  With feat_out
+// (training data) it writes the features and the PP factor of every element.
+template <int EQ, int M, int BE, typename Real, typename TU>
+__device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long long e0,
+                                              const TU (*s_u)[BE * M * M * M], Real (*s_acc)[BE * M * M * M],
+                                              const double* s_p64) {
+    static_assert(M == 4 && BE * M * M == 64, "the learned indicator maps 16 lanes to an element");
+    constexpr int V = NVars<EQ>::V;
+    constexpr int M2 = 16, NN = BE * 64, NT = 64;
+    const int tid = threadIdx.x;
+    const int el = tid / M2;
+    const long long e = e0 + el;
+    const bool valid = e < a.nelem;
+    if (!a.alpha_refresh) return valid ? Real(a.alpha_step[e]) : Real(0);
+
+    Real* q = s_acc[2];
+    // rho, then p; positivity flags per element
+    uint32_t bad = 0;
+    float fr1, fr2, fp1, fp2;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int idx = tid; idx < NN; idx += NT) {
+            const int sl = slot<M>(idx);
+            double rho, p;
+            if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
+                rho = double(s_u[DENS][sl]);
+                p = s_p64[sl];
+            } else {
+                double uu[V];
+#pragma unroll
+                for (int h = 0; h < V; ++h) uu[h] = s_u[h][sl];
+                rho = uu[DENS];
+                p = pressure64<EQ>(a.ph64, uu);
+            }
+            q[sl] = Real(pass == 0 ? rho : p);
+            if (pass == 0 && !(rho > 0.0 && p > 0.0) && e0 + idx / 64 < a.nelem) bad |= 1u << (idx / 64);
+        }
+        __syncthreads();
+        Real tot, c1, c2, s1, s2;
+        modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, c2, s1, s2);
+        tot = sum16(tot);
+        s1 = sum16(s1);
+        s2 = sum16(s2);
+        const float f1 = tot > Real(0) ? float(s1 / tot) : 0.0f;
+        const float f2 = tot > Real(0) ? float(s2 / tot) : 0.0f;
+        if (pass == 0) {
+            fr1 = f1;
+            fr2 = f2;
+        } else {
+            fp1 = f1;
+            fp2 = f2;
+        }
+        __syncthreads();
+    }
+    const float x[4] = {mlp_feature(fr1), mlp_feature(fr2), mlp_feature(fp1), mlp_feature(fp2)};
+    uint32_t bad_el = 0;
+#pragma unroll
+    for (int k = 0; k < BE; ++k)
+        if (__any((bad >> k) & 1u)) bad_el |= 1u << k;
+
+    // judge / fallback / training data need the PP factor
+    const bool judged = valid && (a.feat_out || sample_hash(e, a.step_index) < a.sample_thresh);
+    const bool need_pp = __any(judged) || bad_el != 0;
+    Real a_pp = Real(0);
+    if (need_pp) a_pp = indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64, false);
+
+    if (a.feat_out) {  // training data: features and PP factor of every element
+        if ((tid & 15) == 0 && valid) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a.feat_out[e * 4 + k] = x[k];
+            if (a.alpha_out) a.alpha_out[e] = double(a_pp);
+            a.alpha_step[e] = double(a_pp);
+        }
+        return a_pp;
+    }
+
+    float feat[BE][4], y[BE];
+#pragma unroll
+    for (int k = 0; k < BE; ++k)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) feat[k][f] = lane_bcast(x[f], k * 16);
+    mlp_forward_wave<BE>(a.mlp, feat, y);
+    float ym = y[0];
+#pragma unroll
+    for (int k = 1; k < BE; ++k)
+        if (el == k) ym = y[k];
+    double alpha = double(ym);
+    if (alpha < a.alpha_min)
+        alpha = 0.0;
+    else if (alpha > 1.0 - a.alpha_min)
+        alpha = 1.0;
+    if (alpha > a.alpha_max) alpha = a.alpha_max;
+    int code = 0;  // 1 judged, 2 judged and overridden, 4 fallback
+    if (judged) {
+        code = 1;
+        if (double(a_pp) > alpha + a.judge_tol) {
+            alpha = double(a_pp);
+            code = 3;
+        }
+    }
+    if ((bad_el >> el) & 1u) {
+        alpha = double(a_pp);
+        code |= 4;
+    }
+    if ((tid & 15) == 0 && valid) {
+        a.alpha_step[e] = alpha;
+        if (a.alpha_out) a.alpha_out[e] = alpha;
+        if (a.mlp_stats && code) {
+            if (code & 1) atomicAdd(&a.mlp_stats[0], 1ull);
+            if (code & 2) atomicAdd(&a.mlp_stats[1], 1ull);
+            if (code & 4) atomicAdd(&a.mlp_stats[2], 1ull);
+        }
+    }
+    return Real(alpha);
 }
 
 // Positivity repair of one element held in LDS (one thread; rare path):
@@ -328,7 +500,7 @@ __device__ __forceinline__ void block_cfl(const Phys64& P, const double* __restr
     }
 }
 
-template <int EQ, int M, int BE, int VOL, int SURF, typename Real>
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP = false>
 __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a) {
     using Sh = StageShape<M, BE>;
     constexpr int M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
@@ -384,8 +556,11 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     }
     __syncthreads();
 
-    const Real alpha =
-        a.indicator == IND_PP ? indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64) : Real(0);
+    Real alpha = Real(0);
+    if constexpr (MLP)
+        alpha = indicator_mlp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64);
+    else if (a.indicator == IND_PP)
+        alpha = indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64);
 
     // ---- three direction passes ----------------------------------------
     if constexpr (EQ == EQ_MHD) {

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -247,6 +248,17 @@ struct Solver {
     long long hist_cap = 0;
     hipStream_t stream = nullptr;
 
+    // learned indicator
+    float* mlp_p = nullptr;     // canonical parameters
+    float* mlp_inf = nullptr;   // inference layout
+    float* mlp_m = nullptr;     // Adam moments
+    float* mlp_v = nullptr;
+    unsigned long long* mlp_stats = nullptr;  // judged, overridden, fallback
+    double* alpha_step = nullptr;  // [nelem]
+    bool mlp_ready = false;
+    uint32_t step_counter = 0;
+    double sample_fraction = 0.05, judge_tol = 0.05;
+
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t ev_used = 0;
@@ -264,7 +276,8 @@ void free_mesh_buffers(Solver& s) {
     (void)hipFree(s.nbr);
     (void)hipFree(s.delta);
     (void)hipFree(s.part);
-    s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = s.part = nullptr;
+    (void)hipFree(s.alpha_step);
+    s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = s.part = s.alpha_step = nullptr;
     s.nbr = nullptr;
 }
 
@@ -308,10 +321,24 @@ nemo::Phys64 make_phys64(const nemo_gpu_config& c) {
     return p;
 }
 
+// Per-launch options of the stage kernel.
+struct LaunchOpts {
+    int refresh = 1;            // learned indicator: compute the factor of the step
+    float* feat_out = nullptr;  // training data: features of every element
+};
+
 template <int M, typename Real>
 nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
-                                   double* alpha_out, double* cfl_part) {
+                                   double* alpha_out, double* cfl_part, const LaunchOpts& o) {
     nemo::StageArgs<M, Real> a{};
+    a.mlp = s.mlp_inf;
+    a.alpha_step = s.alpha_step;
+    a.feat_out = o.feat_out;
+    a.mlp_stats = s.mlp_stats;
+    a.alpha_refresh = o.refresh;
+    a.step_index = s.step_counter;
+    a.sample_thresh = s.sample_fraction >= 1.0 ? 0xFFFFFFFFu : uint32_t(s.sample_fraction * 4294967296.0);
+    a.judge_tol = s.judge_tol;
     a.u_in = s.u[s.cur];
     a.u_out = s.u[s.cur ^ 1];
     a.r0 = s.r0;
@@ -354,9 +381,26 @@ void launch_stage_t(Solver& s, const nemo::StageArgs<M, Real>& a) {
 
 template <int M, typename Real>
 int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                    double* cfl_part) {
-    const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
+                    double* cfl_part, const LaunchOpts& o) {
+    const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
+    if constexpr (M == 4) {
+        // learned indicator, and the training-data pass of any Euler scheme
+        if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {
+            constexpr int BE = Tile<M>::BE;
+            const long long nblocks = (s.nelem + BE - 1) / BE;
+            if (o.feat_out || (vol == nemo::VOL_EC && surf == nemo::SURF_HLLC))
+                hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real, true>),
+                                   dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
+            else if (vol == nemo::VOL_EC && surf == nemo::SURF_RUSANOV)
+                hipLaunchKernelGGL(
+                    (nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_RUSANOV, Real, true>),
+                    dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
+            else
+                return fail(NEMO_ERR_ARG, "the learned indicator is built for the EC volume flux with HLLC or Rusanov");
+            return NEMO_OK;
+        }
+    }
 #define NEMO_CASE(E_, V_, S_)                          \
     if (s.cfg.equations == E_ && vol == V_ && surf == S_) { \
         launch_stage_t<E_, M, V_, S_, Real>(s, a);     \
@@ -380,14 +424,16 @@ int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, doubl
 
 template <int M>
 int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                   double* cfl_part) {
+                   double* cfl_part, const LaunchOpts& o) {
     if (s.cfg.precision == 1)
-        return launch_stage_mr<M, float>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
-    return launch_stage_mr<M, double>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part);
+        return launch_stage_mr<M, float>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
+    return launch_stage_mr<M, double>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
 }
 
 int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                 double* cfl_part = nullptr) {
+                 double* cfl_part = nullptr, const LaunchOpts& o = LaunchOpts()) {
+    if (s.cfg.indicator == NEMO_IND_MLP && !s.mlp_ready && !o.feat_out)
+        return fail(NEMO_ERR_NOT_READY, "learned indicator: no network parameters (nemo_gpu_mlp_set_params or _train)");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s.timing) {
         if (s.ev_used == s.ev.size()) {
@@ -403,9 +449,9 @@ int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* 
     }
     int rc;
     switch (s.M) {
-    case 3: rc = launch_stage_m<3>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
-    case 4: rc = launch_stage_m<4>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
-    case 5: rc = launch_stage_m<5>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part); break;
+    case 3: rc = launch_stage_m<3>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o); break;
+    case 4: rc = launch_stage_m<4>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o); break;
+    case 5: rc = launch_stage_m<5>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o); break;
     default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
     }
     if (rc != NEMO_OK) return rc;
@@ -535,7 +581,11 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
             return fail(NEMO_ERR_ARG, "invalid surface flux for GLM-MHD (Rusanov or HLLD)");
         if (!(cfg->mu0 > 0.0)) return fail(NEMO_ERR_ARG, "mu0 must be positive");
     }
-    if (cfg->indicator < 0 || cfg->indicator > 1) return fail(NEMO_ERR_ARG, "invalid indicator");
+    if (cfg->indicator < 0 || cfg->indicator > 2) return fail(NEMO_ERR_ARG, "invalid indicator");
+    if (cfg->indicator == NEMO_IND_MLP &&
+        (cfg->equations != NEMO_EQ_EULER || cfg->n_poly != 3 || cfg->volume != NEMO_VOL_EC ||
+         (cfg->surface != NEMO_SURF_HLLC && cfg->surface != NEMO_SURF_RUSANOV)))
+        return fail(NEMO_ERR_ARG, "the learned indicator is built for Euler, n_poly = 3, EC volume flux, HLLC or Rusanov");
     if (cfg->timedisc < 0 || cfg->timedisc > 1) return fail(NEMO_ERR_ARG, "invalid timedisc");
     if (cfg->precision < 0 || cfg->precision > 1) return fail(NEMO_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 operator)");
     if (!(cfg->gamma > 1.0)) return fail(NEMO_ERR_ARG, "gamma must exceed 1");
@@ -552,6 +602,11 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         return fail(NEMO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     if (hipMalloc(&s->dt_dev, 8 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s->mlp_p, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->mlp_inf, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->mlp_m, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->mlp_v, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->mlp_stats, 3 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
         (void)hipStreamDestroy(s->stream);
         delete s;
@@ -559,6 +614,7 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
     }
     (void)hipMemset(s->status, 0, sizeof(int));
     (void)hipMemset(s->dt_dev, 0, 8 * sizeof(double));
+    (void)hipMemset(s->mlp_stats, 0, 3 * sizeof(unsigned long long));
     *handle = s;
     return NEMO_OK;
 }
@@ -570,6 +626,11 @@ int nemo_gpu_destroy(void* handle) {
     free_mesh_buffers(*s);
     (void)hipFree(s->dt_dev);
     (void)hipFree(s->status);
+    (void)hipFree(s->mlp_p);
+    (void)hipFree(s->mlp_inf);
+    (void)hipFree(s->mlp_m);
+    (void)hipFree(s->mlp_v);
+    (void)hipFree(s->mlp_stats);
     (void)hipFree(s->hist);
     for (auto& p : s->ev) {
         (void)hipEventDestroy(p.first);
@@ -601,6 +662,8 @@ int nemo_gpu_upload_mesh(void* handle, int64_t nelem, const int32_t* nbr, const 
         HIP_TRY(hipMalloc(&s->nbr, size_t(nelem) * 6 * sizeof(int32_t)));
         HIP_TRY(hipMalloc(&s->delta, size_t(nelem) * 3 * sizeof(double)));
         HIP_TRY(hipMalloc(&s->part, size_t(nblocks_for(*s)) * 4 * sizeof(double)));
+        HIP_TRY(hipMalloc(&s->alpha_step, size_t(nelem) * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(s->alpha_step, 0, size_t(nelem) * sizeof(double), s->stream));
         HIP_TRY(hipMemsetAsync(s->r0, 0, bytes, s->stream));
         HIP_TRY(hipMemsetAsync(s->r1, 0, bytes, s->stream));
         s->have_state = false;
@@ -702,7 +765,10 @@ int nemo_gpu_stage(void* handle, int32_t stage, double dt) {
     Solver* s = static_cast<Solver*>(handle);
     if (int rc = check_ready(s)) return rc;
     if (stage < 1 || stage > 5) return fail(NEMO_ERR_ARG, "stage must be 1..5");
-    if (int rc = launch_stage(*s, stage, dt, nullptr, nullptr, nullptr)) return rc;
+    LaunchOpts o;
+    o.refresh = stage == 1;
+    if (int rc = launch_stage(*s, stage, dt, nullptr, nullptr, nullptr, nullptr, o)) return rc;
+    if (stage == 1) ++s->step_counter;
     s->cur ^= 1;
     return NEMO_OK;
 }
@@ -731,7 +797,10 @@ int nemo_gpu_run(void* handle, int32_t nsteps, double* dt_hist) {
         const bool more = it + 1 < nsteps;
         for (int st = 1; st <= 5; ++st) {
             double* part = (st == 5 && more) ? s->part : nullptr;
-            if (int rc = launch_stage(*s, st, 0.0, s->dt_dev, nullptr, nullptr, part)) return rc;
+            LaunchOpts o;
+            o.refresh = st == 1;
+            if (int rc = launch_stage(*s, st, 0.0, s->dt_dev, nullptr, nullptr, part, o)) return rc;
+            if (st == 1) ++s->step_counter;
             s->cur ^= 1;
         }
         if (more)
@@ -757,6 +826,205 @@ int nemo_gpu_get_status(void* handle, int32_t* code) {
     HIP_TRY(hipMemcpyAsync(&c, s->status, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     *code = c;
+    return NEMO_OK;
+}
+
+// ---- learned indicator ------------------------------------------------------
+
+int nemo_gpu_indicator_features(void* handle, float* features, double* alpha_pp) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!features) return fail(NEMO_ERR_ARG, "null argument");
+    if (s->cfg.equations != NEMO_EQ_EULER || s->M != 4)
+        return fail(NEMO_ERR_ARG, "indicator features are built for Euler, n_poly = 3");
+    float* dfeat = nullptr;
+    HIP_TRY(hipMalloc(&dfeat, size_t(s->nelem) * 4 * sizeof(float)));
+    LaunchOpts o;
+    o.feat_out = dfeat;
+    int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, s->alpha, nullptr, o);
+    if (rc == NEMO_OK) {
+        hipError_t e = hipMemcpyAsync(features, dfeat, size_t(s->nelem) * 4 * sizeof(float), hipMemcpyDeviceToHost,
+                                      s->stream);
+        if (e == hipSuccess && alpha_pp)
+            e = hipMemcpyAsync(alpha_pp, s->alpha, size_t(s->nelem) * sizeof(double), hipMemcpyDeviceToHost,
+                               s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = fail(NEMO_ERR_HIP, std::string("indicator features: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dfeat);
+    return rc;
+}
+
+int nemo_gpu_mlp_set_params(void* handle, const float* params) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !params) return fail(NEMO_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(s->mlp_p, params, nemo::MLP_NP * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->mlp_m, 0, nemo::MLP_NP * sizeof(float), s->stream));
+    HIP_TRY(hipMemsetAsync(s->mlp_v, 0, nemo::MLP_NP * sizeof(float), s->stream));
+    hipLaunchKernelGGL(nemo::mlp_inference_layout_kernel, dim3((nemo::MLP_NP + 255) / 256), dim3(256), 0, s->stream,
+                       s->mlp_p, s->mlp_inf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->mlp_ready = true;
+    return NEMO_OK;
+}
+
+int nemo_gpu_mlp_get_params(void* handle, float* params) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !params) return fail(NEMO_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(params, s->mlp_p, nemo::MLP_NP * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_mlp_set_policy(void* handle, double sample_fraction, double tolerance) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    if (!(sample_fraction >= 0.0 && sample_fraction <= 1.0) || !(tolerance >= 0.0))
+        return fail(NEMO_ERR_ARG, "sample fraction must be in [0, 1] and the tolerance >= 0");
+    s->sample_fraction = sample_fraction;
+    s->judge_tol = tolerance;
+    return NEMO_OK;
+}
+
+int nemo_gpu_mlp_stats(void* handle, int64_t* judged, int64_t* overridden, int64_t* fallback) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    unsigned long long h[3];
+    HIP_TRY(hipMemcpyAsync(h, s->mlp_stats, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemsetAsync(s->mlp_stats, 0, sizeof(h), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (judged) *judged = int64_t(h[0]);
+    if (overridden) *overridden = int64_t(h[1]);
+    if (fallback) *fallback = int64_t(h[2]);
+    return NEMO_OK;
+}
+
+int nemo_gpu_mlp_forward(void* handle, int64_t n, const float* features, float* y) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !features || !y || n <= 0) return fail(NEMO_ERR_ARG, "invalid arguments");
+    if (!s->mlp_ready) return fail(NEMO_ERR_NOT_READY, "no network parameters");
+    float *df = nullptr, *dy = nullptr;
+    HIP_TRY(hipMalloc(&df, size_t(n) * 4 * sizeof(float)));
+    if (hipMalloc(&dy, size_t(n) * sizeof(float)) != hipSuccess) {
+        (void)hipFree(df);
+        return fail(NEMO_ERR_HIP, "hipMalloc failed");
+    }
+    hipError_t e = hipMemcpyAsync(df, features, size_t(n) * 4 * sizeof(float), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        const long long groups = (n + 3) / 4;
+        hipLaunchKernelGGL(nemo::mlp_forward_kernel, dim3(unsigned(groups < 65536 ? groups : 65536)), dim3(64), 0,
+                           s->stream, s->mlp_inf, (long long)n, df, dy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dy, size_t(n) * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(df);
+    (void)hipFree(dy);
+    if (e != hipSuccess) return fail(NEMO_ERR_HIP, std::string("mlp forward: ") + hipGetErrorString(e));
+    return NEMO_OK;
+}
+
+namespace {
+
+// One minibatch: partial gradients per block, then the ordered reduction
+// (and an Adam step when lr > 0).
+int mlp_batch(Solver* s, const float* df, const float* dt, const long long* didx, long long off, int B, float* part,
+              float* part_loss, float* dgrad, float* dloss, int nparts, float lr, long long t) {
+    hipLaunchKernelGGL(nemo::mlp_grad_kernel, dim3(nparts), dim3(64), 0, s->stream, s->mlp_p, df, dt, didx, off, B,
+                       part, part_loss);
+    HIP_TRY(hipGetLastError());
+    const float c1 = float(1.0 - std::pow(0.9, double(t))), c2 = float(1.0 - std::pow(0.999, double(t)));
+    hipLaunchKernelGGL(nemo::mlp_reduce_kernel, dim3((nemo::MLP_NP + 255) / 256), dim3(256), 0, s->stream, part,
+                       nparts, part_loss, dgrad, dloss, s->mlp_p, s->mlp_m, s->mlp_v, lr > 0.0f ? 1 : 0, lr, c1, c2);
+    HIP_TRY(hipGetLastError());
+    return NEMO_OK;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+};
+
+}  // namespace
+
+int nemo_gpu_mlp_grad(void* handle, int64_t n, const float* features, const float* target, float* loss,
+                      float* grad) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !features || !target || n <= 0 || n > (1 << 30)) return fail(NEMO_ERR_ARG, "invalid arguments");
+    const int nparts = int(n < 128 ? n : 128);
+    DevBuf f, t, part, pl, g, l;
+    HIP_TRY(hipMalloc(&f.p, size_t(n) * 4 * sizeof(float)));
+    HIP_TRY(hipMalloc(&t.p, size_t(n) * sizeof(float)));
+    HIP_TRY(hipMalloc(&part.p, size_t(nparts) * nemo::MLP_NP * sizeof(float)));
+    HIP_TRY(hipMalloc(&pl.p, size_t(nparts) * sizeof(float)));
+    HIP_TRY(hipMalloc(&g.p, nemo::MLP_NP * sizeof(float)));
+    HIP_TRY(hipMalloc(&l.p, sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(f.p, features, size_t(n) * 4 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(t.p, target, size_t(n) * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(l.p, 0, sizeof(float), s->stream));
+    if (int rc = mlp_batch(s, static_cast<float*>(f.p), static_cast<float*>(t.p), nullptr, 0, int(n),
+                           static_cast<float*>(part.p), static_cast<float*>(pl.p), static_cast<float*>(g.p),
+                           static_cast<float*>(l.p), nparts, 0.0f, 1))
+        return rc;
+    if (grad) HIP_TRY(hipMemcpyAsync(grad, g.p, nemo::MLP_NP * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (loss) HIP_TRY(hipMemcpyAsync(loss, l.p, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_mlp_train(void* handle, int64_t n, const float* features, const float* target, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float* loss_per_epoch) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !features || !target || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0.0f))
+        return fail(NEMO_ERR_ARG, "invalid arguments");
+    if (!s->mlp_ready) return fail(NEMO_ERR_NOT_READY, "set initial parameters first (nemo_gpu_mlp_set_params)");
+    const int nparts = batch < 128 ? batch : 128;
+    DevBuf f, t, idx, part, pl, l;
+    HIP_TRY(hipMalloc(&f.p, size_t(n) * 4 * sizeof(float)));
+    HIP_TRY(hipMalloc(&t.p, size_t(n) * sizeof(float)));
+    HIP_TRY(hipMalloc(&idx.p, size_t(n) * sizeof(long long)));
+    HIP_TRY(hipMalloc(&part.p, size_t(nparts) * nemo::MLP_NP * sizeof(float)));
+    HIP_TRY(hipMalloc(&pl.p, size_t(nparts) * sizeof(float)));
+    HIP_TRY(hipMalloc(&l.p, size_t(epochs) * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(f.p, features, size_t(n) * 4 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(t.p, target, size_t(n) * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(l.p, 0, size_t(epochs) * sizeof(float), s->stream));
+    std::vector<long long> perm(static_cast<size_t>(n));
+    for (long long i = 0; i < n; ++i) perm[size_t(i)] = i;
+    uint64_t state = seed ^ 0x9E3779B97F4A7C15ull;
+    auto next = [&state]() {  // SplitMix64
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    long long t_step = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (long long i = n - 1; i > 0; --i) std::swap(perm[size_t(i)], perm[size_t(next() % uint64_t(i + 1))]);
+        HIP_TRY(hipMemcpyAsync(idx.p, perm.data(), size_t(n) * sizeof(long long), hipMemcpyHostToDevice, s->stream));
+        for (long long off = 0; off + batch <= n; off += batch) {
+            ++t_step;
+            if (int rc = mlp_batch(s, static_cast<float*>(f.p), static_cast<float*>(t.p),
+                                   static_cast<long long*>(idx.p), off, batch, static_cast<float*>(part.p),
+                                   static_cast<float*>(pl.p), nullptr, static_cast<float*>(l.p) + ep, nparts, lr,
+                                   t_step))
+                return rc;
+        }
+        // the host permutation buffer is rewritten next epoch: wait for this one
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    if (loss_per_epoch) {
+        std::vector<float> h(static_cast<size_t>(epochs));
+        HIP_TRY(hipMemcpyAsync(h.data(), l.p, size_t(epochs) * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        const long long nb = n / batch;
+        for (int ep = 0; ep < epochs; ++ep) loss_per_epoch[ep] = nb > 0 ? h[size_t(ep)] / float(nb) : 0.0f;
+    }
+    hipLaunchKernelGGL(nemo::mlp_inference_layout_kernel, dim3((nemo::MLP_NP + 255) / 256), dim3(256), 0, s->stream,
+                       s->mlp_p, s->mlp_inf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return NEMO_OK;
 }
 

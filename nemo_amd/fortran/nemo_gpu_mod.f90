@@ -20,13 +20,17 @@ integer(c_int), parameter :: NEMO_ERR_STATE     = 3
 integer(c_int), parameter :: NEMO_ERR_NOT_READY = 4
 
 integer(c_int32_t), parameter :: NEMO_EQ_EULER     = 0
+integer(c_int32_t), parameter :: NEMO_EQ_GLM_MHD   = 1
 integer(c_int32_t), parameter :: NEMO_VOL_WEAK     = 0
 integer(c_int32_t), parameter :: NEMO_VOL_CENTRAL  = 1
 integer(c_int32_t), parameter :: NEMO_VOL_EC       = 2
 integer(c_int32_t), parameter :: NEMO_SURF_RUSANOV = 0
 integer(c_int32_t), parameter :: NEMO_SURF_HLLC    = 1
+integer(c_int32_t), parameter :: NEMO_SURF_HLLD    = 2
 integer(c_int32_t), parameter :: NEMO_IND_NONE     = 0
 integer(c_int32_t), parameter :: NEMO_IND_PP       = 1
+integer(c_int32_t), parameter :: NEMO_IND_MLP      = 2
+integer(c_int32_t), parameter :: NEMO_MLP_NPARAMS  = 8705
 integer(c_int32_t), parameter :: NEMO_TD_LSRK54    = 0
 integer(c_int32_t), parameter :: NEMO_TD_SSPRK54   = 1
 
@@ -105,6 +109,91 @@ interface
         type(c_ptr), value          :: handle
         real(c_double), intent(out) :: dt, lmax, vmax
         integer(c_int)              :: rc
+    end function
+
+    !! equations_share_mod::glm_ch / glm_alpha (GLM-MHD)
+    function nemo_gpu_set_glm(handle, ch, alpha) result(rc) bind(c, name='nemo_gpu_set_glm')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value    :: handle
+        real(c_double), value :: ch, alpha
+        integer(c_int)        :: rc
+    end function
+
+    function nemo_gpu_get_glm(handle, ch, alpha) result(rc) bind(c, name='nemo_gpu_get_glm')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value          :: handle
+        real(c_double), intent(out) :: ch, alpha
+        integer(c_int)              :: rc
+    end function
+
+    !! learned troubled-cell indicator
+    function nemo_gpu_mlp_set_params(handle, params) result(rc) bind(c, name='nemo_gpu_mlp_set_params')
+        import :: c_int, c_ptr, c_float
+        type(c_ptr), value         :: handle
+        real(c_float), intent(in)  :: params(*)
+        integer(c_int)             :: rc
+    end function
+
+    function nemo_gpu_mlp_get_params(handle, params) result(rc) bind(c, name='nemo_gpu_mlp_get_params')
+        import :: c_int, c_ptr, c_float
+        type(c_ptr), value         :: handle
+        real(c_float), intent(out) :: params(*)
+        integer(c_int)             :: rc
+    end function
+
+    function nemo_gpu_mlp_set_policy(handle, sample_fraction, tolerance) result(rc) &
+            bind(c, name='nemo_gpu_mlp_set_policy')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value    :: handle
+        real(c_double), value :: sample_fraction, tolerance
+        integer(c_int)        :: rc
+    end function
+
+    function nemo_gpu_mlp_stats(handle, judged, overridden, fallback) result(rc) bind(c, name='nemo_gpu_mlp_stats')
+        import :: c_int, c_ptr, c_int64_t
+        type(c_ptr), value              :: handle
+        integer(c_int64_t), intent(out) :: judged, overridden, fallback
+        integer(c_int)                  :: rc
+    end function
+
+    function nemo_gpu_indicator_features(handle, features, alpha_pp) result(rc) &
+            bind(c, name='nemo_gpu_indicator_features')
+        import :: c_int, c_ptr, c_float, c_double
+        type(c_ptr), value          :: handle
+        real(c_float), intent(out)  :: features(*)
+        real(c_double), intent(out) :: alpha_pp(*)
+        integer(c_int)              :: rc
+    end function
+
+    function nemo_gpu_mlp_forward(handle, n, features, y) result(rc) bind(c, name='nemo_gpu_mlp_forward')
+        import :: c_int, c_ptr, c_int64_t, c_float
+        type(c_ptr), value         :: handle
+        integer(c_int64_t), value  :: n
+        real(c_float), intent(in)  :: features(*)
+        real(c_float), intent(out) :: y(*)
+        integer(c_int)             :: rc
+    end function
+
+    function nemo_gpu_mlp_grad(handle, n, features, target, loss, grad) result(rc) bind(c, name='nemo_gpu_mlp_grad')
+        import :: c_int, c_ptr, c_int64_t, c_float
+        type(c_ptr), value         :: handle
+        integer(c_int64_t), value  :: n
+        real(c_float), intent(in)  :: features(*), target(*)
+        real(c_float), intent(out) :: loss, grad(*)
+        integer(c_int)             :: rc
+    end function
+
+    function nemo_gpu_mlp_train(handle, n, features, target, epochs, batch, lr, seed, loss_per_epoch) result(rc) &
+            bind(c, name='nemo_gpu_mlp_train')
+        import :: c_int, c_ptr, c_int64_t, c_int32_t, c_float
+        type(c_ptr), value         :: handle
+        integer(c_int64_t), value  :: n
+        real(c_float), intent(in)  :: features(*), target(*)
+        integer(c_int32_t), value  :: epochs, batch
+        real(c_float), value       :: lr
+        integer(c_int64_t), value  :: seed
+        real(c_float), intent(out) :: loss_per_epoch(*)
+        integer(c_int)             :: rc
     end function
 
     function nemo_gpu_stage(handle, stage, dt) result(rc) bind(c, name='nemo_gpu_stage')

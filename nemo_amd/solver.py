@@ -24,7 +24,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import Config, NemoGpuError, check, dptr, iptr, lib
+from ._lib import Config, NemoGpuError, check, dptr, fptr, iptr, lib
 from .mesh import Mesh
 
 # timedisc_const_mod::dt_coeffs (low-storage/4-5:29 = b_rk, ssp/4-5:36)
@@ -147,6 +147,56 @@ class Solver:
         ch, alpha = ctypes.c_double(), ctypes.c_double()
         self._call("nemo_gpu_get_glm", self._h, ctypes.byref(ch), ctypes.byref(alpha))
         return ch.value, alpha.value
+
+    # -- learned indicator ------------------------------------------------------
+    def indicator_features(self):
+        """Features [nelem, 4] (float32) and Persson-Peraire factors of the
+        current state: the training data of the learned indicator."""
+        f = np.empty((self.nelem, 4), dtype=np.float32)
+        a = np.empty(self.nelem)
+        self._call("nemo_gpu_indicator_features", self._h, fptr(f), dptr(a))
+        return f, a
+
+    def mlp_set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == _lib.MLP_NPARAMS
+        self._call("nemo_gpu_mlp_set_params", self._h, fptr(p))
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.empty(_lib.MLP_NPARAMS, dtype=np.float32)
+        self._call("nemo_gpu_mlp_get_params", self._h, fptr(p))
+        return p
+
+    def mlp_set_policy(self, sample_fraction: float = 0.05, tolerance: float = 0.05) -> None:
+        self._call("nemo_gpu_mlp_set_policy", self._h, ctypes.c_double(sample_fraction), ctypes.c_double(tolerance))
+
+    def mlp_stats(self):
+        j, o, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._call("nemo_gpu_mlp_stats", self._h, ctypes.byref(j), ctypes.byref(o), ctypes.byref(f))
+        return j.value, o.value, f.value
+
+    def mlp_forward(self, feat: np.ndarray) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        y = np.empty(feat.shape[0], dtype=np.float32)
+        self._call("nemo_gpu_mlp_forward", self._h, ctypes.c_int64(feat.shape[0]), fptr(feat), fptr(y))
+        return y
+
+    def mlp_grad(self, feat: np.ndarray, target: np.ndarray):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(_lib.MLP_NPARAMS, dtype=np.float32)
+        self._call("nemo_gpu_mlp_grad", self._h, ctypes.c_int64(feat.shape[0]), fptr(feat), fptr(t), fptr(loss), fptr(g))
+        return float(loss[0]), g
+
+    def mlp_train(self, feat: np.ndarray, target: np.ndarray, epochs: int = 10, batch: int = 4096,
+                  lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        self._call("nemo_gpu_mlp_train", self._h, ctypes.c_int64(feat.shape[0]), fptr(feat), fptr(t),
+                   ctypes.c_int32(epochs), ctypes.c_int32(batch), ctypes.c_float(lr), ctypes.c_uint64(seed), fptr(hist))
+        return hist
 
     # -- timedisc interface ----------------------------------------------------
     def dt_coeffs(self):

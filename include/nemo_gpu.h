@@ -178,6 +178,31 @@ int nemo_gpu_mlp_grad(void *handle, int64_t n, const float *features, const floa
 int nemo_gpu_mlp_train(void *handle, int64_t n, const float *features, const float *target, int32_t epochs,
                        int32_t batch, float lr, uint64_t seed, float *loss_per_epoch);
 
+/* ---- batched ensemble ----------------------------------------------------
+ * Independent runs of one scheme advanced together (the reference runs one
+ * mesh per process; this has no counterpart there).  Every run has its own
+ * mesh, state, n_poly (2..4) and CFL time step; runs of equal n_poly share
+ * one persistent launch per RK stage, and pairs of time steps are replayed
+ * as a captured HIP graph.  Built for Euler with the EC volume flux, HLLC,
+ * no indicator or Persson-Peraire; cfg->n_poly is not used. */
+int nemo_gpu_batch_create(const nemo_gpu_config *cfg, int32_t nruns, const int32_t *n_poly,
+                          const int64_t *nelem, void **handle);
+int nemo_gpu_batch_destroy(void *handle);
+/* Mesh (as nemo_gpu_upload_mesh, run-local element ids) and state of a run. */
+int nemo_gpu_batch_upload(void *handle, int32_t run, const int32_t *nbr, const double *delta, const double *u);
+int nemo_gpu_batch_download(void *handle, int32_t run, double *u);
+/* nsteps CFL-limited steps of every run; dt_hist[nsteps][nruns] may be NULL.
+ * Synchronises; NEMO_ERR_STATE if a repair failed. */
+int nemo_gpu_batch_run(void *handle, int32_t nsteps, int32_t use_graph, double *dt_hist);
+/* Launch geometry: grid_per_cu workgroups per CU for the persistent stage
+ * launches (0: one workgroup per block of elements) and the XCD-contiguous
+ * block map on or off. */
+int nemo_gpu_batch_set_launch(void *handle, int32_t grid_per_cu, int32_t xcd_map);
+/* Times `steps` steps of every candidate geometry on the current states
+ * (which are restored afterwards), keeps the fastest and reports it. */
+int nemo_gpu_batch_autotune(void *handle, int32_t steps, int32_t *grid_per_cu, int32_t *xcd_map,
+                            double *best_ms_per_step);
+
 /* Operators for polynomial degree N (host only, no device needed):
  * LGL nodes x[M], weights w[M] (utils/interpol_mod.f90:513-582),
  * D[M][M] with D[i][j] = l_j'(x_i) (interpol_mod.f90:38-80), and the

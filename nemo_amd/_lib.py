@@ -81,6 +81,13 @@ SIGNATURES = {
     "nemo_gpu_mlp_grad": [_VP, ctypes.c_int64, _PF, _PF, _PF, _PF],
     "nemo_gpu_mlp_train": [_VP, ctypes.c_int64, _PF, _PF, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                            ctypes.c_uint64, _PF],
+    "nemo_gpu_batch_create": [ctypes.POINTER(Config), ctypes.c_int32, _PI32, _PI64, ctypes.POINTER(_VP)],
+    "nemo_gpu_batch_destroy": [_VP],
+    "nemo_gpu_batch_upload": [_VP, ctypes.c_int32, _PI32, _P, _P],
+    "nemo_gpu_batch_download": [_VP, ctypes.c_int32, _P],
+    "nemo_gpu_batch_run": [_VP, ctypes.c_int32, ctypes.c_int32, _P],
+    "nemo_gpu_batch_set_launch": [_VP, ctypes.c_int32, ctypes.c_int32],
+    "nemo_gpu_batch_autotune": [_VP, ctypes.c_int32, _PI32, _PI32, _P],
     "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
     "nemo_gpu_read_timing": [_VP, _PI64, _P],
     "nemo_gpu_operators": [ctypes.c_int32, _P, _P, _P, _P],

@@ -28,14 +28,27 @@ def _run(cmd, cwd=ROOT):
     return out
 
 
+HIP_SOURCES = ["nemo_gpu.hip", "batch.hip"]
+
+
 def build_hip(verbose: bool = False) -> str:
-    src = os.path.join(HERE, "csrc", "nemo_gpu.hip")
+    """Compile the HIP translation units in parallel and link libnemo_gpu.so."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(HERE, "libnemo_gpu.so")
+    bdir = os.path.join(HERE, "csrc", "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = [os.path.join(bdir, s.replace(".hip", ".o")) for s in HIP_SOURCES]
+
+    def compile_one(k):
+        return _run([HIPCC, *HIP_FLAGS, "-c", "-o", objs[k], os.path.join(HERE, "csrc", HIP_SOURCES[k])])
+
+    with ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
+        logs = list(ex.map(compile_one, range(len(HIP_SOURCES))))
     tmp = out + ".tmp"
-    log = _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    logs.append(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, *objs]))
     os.replace(tmp, out)
-    if verbose and log:
-        print(log)
+    if verbose:
+        print("\n".join(l for l in logs if l))
     return out
 
 

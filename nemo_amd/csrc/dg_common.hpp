@@ -78,6 +78,8 @@ struct StageArgs {
     const double* __restrict__ delta;// [nelem][3]
     const double* __restrict__ dt_dev;
     const double* __restrict__ glm;  // MHD: [c_h, alpha] of the current step
+    const double* __restrict__ dt_run;     // batched ensemble: time step per run
+    const int32_t* __restrict__ block_run; // batched ensemble: run of each block
     int* __restrict__ status;
     double* __restrict__ cfl_part;   // non-null: per-block CFL partials of the new state
     // learned indicator

@@ -500,30 +500,40 @@ __device__ __forceinline__ void block_cfl(const Phys64& P, const double* __restr
     }
 }
 
-template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP = false>
-__global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a) {
+// LDS of one stage block.  FP32 Euler operator: the operator reads an FP32
+// copy of the state (plus the FP64 pressure per node for the HLLC/HLLE
+// blend) and the FP64 new state of the RK update is laid over the
+// operator's LDS afterwards, so a workgroup needs 12 KB instead of 17.5 KB
+// and more of them fit on a CU.
+template <int EQ, int M, int BE, typename Real>
+struct StageLds {
+    static constexpr int NN = BE * M * M * M, V = NVars<EQ>::V;
+    static constexpr bool COMPACT = sizeof(Real) == 4 && EQ == EQ_EULER;
+    using TU = typename std::conditional<COMPACT, float, double>::type;
+    static constexpr int OP_DOUBLES = (V * NN * int(sizeof(TU)) + V * NN * int(sizeof(Real)) + 7) / 8;
+    static constexpr int RAW_DOUBLES = OP_DOUBLES > V * NN ? OP_DOUBLES : V * NN;
+    static constexpr int DOUBLES = RAW_DOUBLES + (COMPACT ? NN : 1);
+};
+
+// One block of BE elements through a whole stage (the body of the stage
+// kernels); lds holds StageLds::DOUBLES doubles.
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP>
+__device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const long long blk, double* lds) {
     using Sh = StageShape<M, BE>;
+    using L = StageLds<EQ, M, BE, Real>;
     constexpr int M3 = Sh::M3, NN = Sh::NN, NT = Sh::NT;
     static_assert(NT <= 64, "a workgroup is one wavefront");
     constexpr int V = NVars<EQ>::V;
     constexpr bool F32 = sizeof(Real) == 4;
-    // FP32 Euler operator: the operator reads an FP32 copy of the state (plus
-    // the FP64 pressure per node for the HLLC/HLLE blend) and the FP64 new
-    // state of the RK update is laid over the operator's LDS afterwards, so a
-    // workgroup needs 12 KB instead of 17.5 KB and more of them fit on a CU.
-    constexpr bool COMPACT = F32 && EQ == EQ_EULER;
-    using TU = typename std::conditional<COMPACT, float, double>::type;
-    constexpr int OP_DOUBLES = (V * NN * int(sizeof(TU)) + V * NN * int(sizeof(Real)) + 7) / 8;
-    constexpr int RAW_DOUBLES = OP_DOUBLES > V * NN ? OP_DOUBLES : V * NN;
-    __shared__ double s_raw[RAW_DOUBLES];
-    __shared__ double s_p64[COMPACT ? NN : 1];
+    constexpr bool COMPACT = L::COMPACT;
+    using TU = typename L::TU;
+    double* s_raw = lds;
+    double* s_p64 = lds + L::RAW_DOUBLES;
     TU (*s_u)[NN] = reinterpret_cast<TU (*)[NN]>(&s_raw[0]);
     Real (*s_acc)[NN] = reinterpret_cast<Real (*)[NN]>(reinterpret_cast<TU*>(&s_raw[0]) + V * NN);
     double (*s_u64)[NN] = reinterpret_cast<double (*)[NN]>(&s_raw[0]);  // FP64 state (RK update, repair, CFL)
 
     const int tid = threadIdx.x;
-    const long long nblocks = (a.nelem + BE - 1) / BE;
-    const long long blk = xcd_block(nblocks);
     const long long e0 = blk * BE;
     const long long EM3 = a.nelem * (long long)M3;
     const long long nvalid_nodes = (a.nelem - e0 < BE ? a.nelem - e0 : BE) * (long long)M3;
@@ -599,7 +609,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
 
     // ---- RK probe: LSRK 5-4 (low-storage/4-5/timedisc_probe_mod.f90:31-37)
     //      or SSPRK(5,4) (ssp/4-5/timedisc_probe_mod.f90:24-46) ---------------
-    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
     const int s = a.stage;
     auto probe = [&](long long gi, double u, double rhs) {
         double st;
@@ -697,6 +707,41 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a)
     if (a.cfl_part) block_cfl<EQ, M, BE, NT>(a.ph64, a.delta, e0, a.nelem, s_u64, &s_raw[0], a.cfl_part + 4 * blk);
 }
 
+// One workgroup per block of BE elements.
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP = false>
+__global__ __launch_bounds__(BE * M * M) void stage_kernel(StageArgs<M, Real> a) {
+    __shared__ double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
+    const long long nblocks = (a.nelem + BE - 1) / BE;
+    stage_block<EQ, M, BE, VOL, SURF, Real, MLP>(a, xcd_block(nblocks), lds);
+}
+
+// Persistent form for the batched ensemble: a grid of gridDim.x workgroups
+// walks all blocks; with xcd != 0 the walk is dealt so that each XCD (the
+// hardware deals workgroups round-robin over the 8 XCDs) takes a contiguous
+// stretch of blocks.
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real>
+__global__ __launch_bounds__(BE * M * M) void stage_kernel_persistent(StageArgs<M, Real> a, int xcd) {
+    __shared__ double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
+    const long long nblocks = (a.nelem + BE - 1) / BE;
+    const long long G = gridDim.x;
+    const long long per = (nblocks + G - 1) / G;
+    for (long long it = 0; it < per; ++it) {
+        long long b;
+        if (xcd && (G & 7) == 0) {
+            // workgroup w sits on XCD w % 8; XCD x owns blocks [x*nb8, (x+1)*nb8)
+            const long long w = blockIdx.x, x = w & 7, lw = w >> 3, G8 = G >> 3;
+            const long long nb8 = (nblocks + 7) / 8;
+            b = x * nb8 + it * G8 + lw;
+            if (b >= (x + 1) * nb8) continue;
+        } else {
+            b = it * G + blockIdx.x;
+        }
+        if (b >= nblocks) continue;
+        stage_block<EQ, M, BE, VOL, SURF, Real, false>(a, b, lds);
+        __syncthreads();
+    }
+}
+
 // Standalone CFL reduction of a state in HBM (first step, calc_timestep):
 // the same per-block partials as the fused stage epilogue.
 template <int EQ, int M, int BE>
@@ -724,7 +769,7 @@ __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restric
 // GLM-MHD also c_h = 2|lmax - vmax| and alpha = c_h/0.18
 // (kernel/mcfv/3d/kernel_timestep_mod.f90:53-56).
 // dt_dev: [0] dt, [1] lmax, [2] vmax, [3] c_h, [4] alpha.
-__global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ part, long long npart, double cfl,
+static __global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ part, long long npart, double cfl,
                                                       int M, double* __restrict__ dt_dev, double* __restrict__ hist,
                                                       long long slot) {
     __shared__ double s[3][1024];

@@ -110,7 +110,7 @@ __device__ __forceinline__ void mlp_forward_wave(const float* __restrict__ P, co
 
 // Forward of n samples for the host API (inference layout): each wavefront
 // takes groups of 4 samples.
-__global__ __launch_bounds__(64) void mlp_forward_kernel(const float* __restrict__ P, long long n,
+static __global__ __launch_bounds__(64) void mlp_forward_kernel(const float* __restrict__ P, long long n,
                                                          const float* __restrict__ feat, float* __restrict__ out) {
     const long long groups = (n + 3) / 4;
     for (long long g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64) void mlp_forward_kernel(const float* __restrict
 // mlp_reduce_kernel sums the partials in a fixed order (deterministic).
 // The weights are staged in LDS (rows padded to 65 floats so that both
 // row and column reads are conflict-free).
-__global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ P, const float* __restrict__ feat,
+static __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ P, const float* __restrict__ feat,
                                                       const float* __restrict__ target,
                                                       const long long* __restrict__ idx, long long off, int B,
                                                       float* __restrict__ part, float* __restrict__ part_loss) {
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* __restrict__ 
 // Sum of the per-block partial gradients in block order (deterministic);
 // with adam != 0 also one Adam step (Kingma & Ba 2015) on P (canonical) with
 // moments m, v and bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
-__global__ void mlp_reduce_kernel(const float* __restrict__ part, int nparts, const float* __restrict__ part_loss,
+static __global__ void mlp_reduce_kernel(const float* __restrict__ part, int nparts, const float* __restrict__ part_loss,
                                   float* __restrict__ grad, float* __restrict__ loss, float* __restrict__ P,
                                   float* __restrict__ m, float* __restrict__ v, int adam, float lr, float c1, float c2) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,7 +241,7 @@ __global__ void mlp_reduce_kernel(const float* __restrict__ part, int nparts, co
 }
 
 // Canonical -> inference layout (W2 and W3 transposed).
-__global__ void mlp_inference_layout_kernel(const float* __restrict__ P, float* __restrict__ Q) {
+static __global__ void mlp_inference_layout_kernel(const float* __restrict__ P, float* __restrict__ Q) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= MLP_NP) return;
     if (p >= MLP_W2 && p < MLP_B2) {

@@ -119,6 +119,46 @@ def orszag_tang(mesh: Mesh, N: int, gamma: float = 5.0 / 3.0, seed: int = 1234, 
     return np.ascontiguousarray(mhd_prim2cons(rho, v, p, B, np.zeros_like(x), gamma))
 
 
+def planar_riemann(mesh: Mesh, N: int, rng: np.random.Generator, gamma: float = 1.4) -> np.ndarray:
+    """Planar Riemann problem on the periodic unit cube: a plane through the
+    centre with a random normal separates two constant states with
+    rho in [0.125, 1], p in [0.1, 1] and |u| <= 0.5 (random directions)."""
+    xi, _ = operators.lgl(N)
+    X = mesh.node_coords(xi)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    side = ((X - 0.5) * n[None, :, None]).sum(1) < 0.0
+    st = []
+    for _ in range(2):
+        rho = rng.uniform(0.125, 1.0)
+        p = rng.uniform(0.1, 1.0)
+        d = rng.normal(size=3)
+        v = d / np.linalg.norm(d) * rng.uniform(0.0, 0.5)
+        st.append((rho, v, p))
+    rho = np.where(side, st[0][0], st[1][0])
+    vx = np.where(side, st[0][1][0], st[1][1][0])
+    vy = np.where(side, st[0][1][1], st[1][1][1])
+    vz = np.where(side, st[0][1][2], st[1][1][2])
+    p = np.where(side, st[0][2], st[1][2])
+    return np.ascontiguousarray(prim2cons(rho, vx, vy, vz, p, gamma))
+
+
+def ensemble_c5(nruns: int = 256, seed: int = 1234, nmin: int = 8, nmax: int = 32):
+    """Config 5: `nruns` independent planar Riemann problems, elements per
+    direction uniform in [nmin, nmax], N uniform in {2, 3, 4}.  Returns
+    (meshes, n_polys, states)."""
+    rng = np.random.default_rng(seed)
+    meshes, npolys, states = [], [], []
+    for _ in range(nruns):
+        n = int(rng.integers(nmin, nmax + 1))
+        N = int(rng.integers(2, 5))
+        m = uniform_periodic(n)
+        meshes.append(m)
+        npolys.append(N)
+        states.append(planar_riemann(m, N, rng))
+    return meshes, npolys, states
+
+
 def config_c1(n: int = 8, N: int = 3):
     """Config 1: 8^3 periodic unit cube, N=3, isentropic vortex, FP64."""
     mesh = uniform_periodic(n)

@@ -249,3 +249,71 @@ class Solver:
             rt.simtime += rt.timestep
             rt.timesteps += 1
         return rt
+
+
+class Batch:
+    """Batched ensemble of independent runs (nemo_gpu_batch_*): each run has
+    its own mesh, state, polynomial degree and CFL time step."""
+
+    def __init__(self, cfg: Config, meshes, n_polys):
+        self.cfg = cfg
+        self.meshes = list(meshes)
+        self.n_polys = [int(n) for n in n_polys]
+        self.nruns = len(self.meshes)
+        npoly = np.array(self.n_polys, dtype=np.int32)
+        nel = np.array([m.nelem for m in self.meshes], dtype=np.int64)
+        self._h = ctypes.c_void_p()
+        self._call("nemo_gpu_batch_create", ctypes.byref(cfg), ctypes.c_int32(self.nruns), iptr(npoly),
+                   nel.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(self._h))
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), name)(*args)
+        if rc != _lib.NEMO_OK:
+            raise NemoGpuError(rc, name, _lib.last_error())
+
+    def close(self):
+        if self._h:
+            lib().nemo_gpu_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, run: int, u: np.ndarray) -> None:
+        m = self.meshes[run]
+        M3 = (self.n_polys[run] + 1) ** 3
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (m.nelem, 5, M3), u.shape
+        nbr = np.ascontiguousarray(m.nbr, dtype=np.int32)
+        delta = np.ascontiguousarray(m.delta, dtype=np.float64)
+        self._call("nemo_gpu_batch_upload", self._h, ctypes.c_int32(run), iptr(nbr), dptr(delta), dptr(u))
+
+    def download(self, run: int) -> np.ndarray:
+        m = self.meshes[run]
+        u = np.empty((m.nelem, 5, (self.n_polys[run] + 1) ** 3))
+        self._call("nemo_gpu_batch_download", self._h, ctypes.c_int32(run), dptr(u))
+        return u
+
+    def run(self, nsteps: int, use_graph: bool = True, record_dt: bool = False):
+        hist = np.zeros((max(nsteps, 1), self.nruns)) if record_dt else None
+        self._call("nemo_gpu_batch_run", self._h, ctypes.c_int32(nsteps), ctypes.c_int32(1 if use_graph else 0),
+                   dptr(hist) if record_dt else None)
+        return hist[:nsteps] if record_dt else None
+
+    def set_launch(self, grid_per_cu: int, xcd_map: bool = True) -> None:
+        self._call("nemo_gpu_batch_set_launch", self._h, ctypes.c_int32(grid_per_cu), ctypes.c_int32(int(xcd_map)))
+
+    def autotune(self, steps: int = 2):
+        g, x, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        self._call("nemo_gpu_batch_autotune", self._h, ctypes.c_int32(steps), ctypes.byref(g), ctypes.byref(x),
+                   ctypes.byref(ms))
+        return g.value, bool(x.value), ms.value

@@ -163,27 +163,46 @@ def test_sampled_judge_fraction():
 
 @pytest.mark.skipif(not os.path.exists(indicator_mlp.DEFAULT_PARAMS), reason="no trained parameters")
 def test_trained_network_gain_on_sedov():
-    """The trained network on a Sedov blast it was not trained on (new
-    centre): it agrees with the Persson-Peraire factor on the elements it
-    blends, and the FP32 run with the learned indicator stays within the
-    plain indicator's solution, while pure DG without an indicator leaves the
-    repair to hold the solution together (many more repaired elements)."""
-    m = meshmod.uniform_periodic(16)
-    u0 = indicator_mlp.sedov(m, 3, seed=987)
+    """The trained network on Sedov blasts it was not trained on (new
+    centres, 16^3 and 24^3), FP32 operator, 80 steps.  Against the method
+    without it (pure DG, no indicator) it keeps the post-shock density
+    undershoot of the plain indicator (min rho ~ 5e-2 instead of ~1e-3) and
+    the plain indicator's solution (L1 density difference < 1e-4, against
+    ~3e-3 for pure DG), while the Persson-Peraire indicator only runs on the
+    sampled 5% of elements per step."""
     params = indicator_mlp.load_params()
-    res = {}
-    for name, ind in (("pp", _lib.IND_PP), ("mlp", _lib.IND_MLP), ("none", _lib.IND_NONE)):
-        with solver.Solver(_cfg(ind=ind, precision=1)) as s:
-            s.upload_mesh(m)
-            s.set_state(u0)
-            if ind == _lib.IND_MLP:
-                s.mlp_set_params(params)
-            try:
-                s.run(60)
-                res[name] = s.get_state()
-            except _lib.NemoGpuError as e:
-                res[name] = e
-    assert isinstance(res["pp"], np.ndarray) and isinstance(res["mlp"], np.ndarray)
-    d = np.abs(res["mlp"][:, 0] - res["pp"][:, 0]).sum() / np.abs(res["pp"][:, 0]).sum()
-    print("L1 density mlp vs pp", d)
-    assert d < 1e-2
+    for n, seed in ((16, 987), (24, 4321)):
+        m = meshmod.uniform_periodic(n)
+        u0 = indicator_mlp.sedov(m, 3, seed=seed)
+        res, alphas = {}, {}
+        for name, ind in (("pp", _lib.IND_PP), ("mlp", _lib.IND_MLP), ("none", _lib.IND_NONE)):
+            with solver.Solver(_cfg(ind=ind, precision=1)) as s:
+                s.upload_mesh(m)
+                s.set_state(u0)
+                if ind == _lib.IND_MLP:
+                    s.mlp_set_params(params)
+                try:
+                    s.run(80)
+                    res[name] = s.get_state()
+                    alphas[name] = s.indicator()
+                    if ind == _lib.IND_MLP:
+                        print(f"  {n}^3 judge stats", s.mlp_stats())
+                except _lib.NemoGpuError as e:
+                    res[name] = e
+            print(f"  {n}^3 {name}: {'ok' if isinstance(res[name], np.ndarray) else res[name]}")
+        assert isinstance(res["pp"], np.ndarray) and isinstance(res["mlp"], np.ndarray)
+        d = np.abs(res["mlp"][:, 0] - res["pp"][:, 0]).sum() / np.abs(res["pp"][:, 0]).sum()
+        a_pp, a_mlp = alphas["pp"], alphas["mlp"]
+        print(f"  {n}^3 L1 density mlp vs pp {d:.3e}; blended pp {np.mean(a_pp > 0):.4f} mlp {np.mean(a_mlp > 0):.4f};"
+              f" max underestimate {np.max(a_pp - a_mlp):.3f}")
+        assert d < 1e-4
+        assert np.max(a_pp - a_mlp) < 0.15
+        rmin = {k: float(res[k][:, 0].min()) for k in ("pp", "mlp")}
+        assert rmin["mlp"] >= 0.9 * rmin["pp"]
+        if isinstance(res["none"], np.ndarray):
+            dn = np.abs(res["none"][:, 0] - res["pp"][:, 0]).sum() / np.abs(res["pp"][:, 0]).sum()
+            rn = float(res["none"][:, 0].min())
+            print(f"  {n}^3 L1 density none vs pp {dn:.3e}, min rho none {rn:.3e}"
+                  f" pp {rmin['pp']:.3e} mlp {rmin['mlp']:.3e}")
+            assert d < 0.05 * dn
+            assert rmin["mlp"] > 5.0 * rn

@@ -35,6 +35,7 @@ Prints one JSON line; `value` is the plain FP64 path.  Extra objects:
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -227,6 +228,9 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with one plain device copy of 1 GiB: the known-size kernel that scales "
+                         "FETCH_SIZE/WRITE_SIZE in counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -249,8 +253,15 @@ def main():
     assert launches == stage_launches, (launches, stage_launches)
     avg_launch_s = kernel_ms / 1e3 / launches
     achieved = total_bytes / (kernel_ms / 1e3) / 1e9
+    # HBM bytes per launch from the calibrated FETCH_SIZE/WRITE_SIZE counter
+    # runs of this workload (profiles/traffic_stage_kernel_kh64_n3.json)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_stage_kernel_kh64_n3.json")
+    if os.path.exists(tpath) and args.elems == 64:
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "stage_kernel<4,4,EC,HLLC,double>", "avg_launch_us": round(avg_launch_s * 1e6, 2),
                 "bytes_per_launch_avg": int(bytes_per_step / NSTAGES)}
 
@@ -310,6 +321,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.elems, N, args.seed, args.cpu_steps, cfg_kw)
     print(json.dumps(out), flush=True)
+    if args.calibrate:
+        from nemo_amd import solver
+        with solver.Solver(solver.make_config(N=N)) as s:
+            s._call("nemo_gpu_copy_calibration", s._h, ctypes.c_int64(1 << 30))
 
 
 if __name__ == "__main__":

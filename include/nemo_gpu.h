@@ -143,6 +143,9 @@ int nemo_gpu_get_status(void *handle, int32_t *code);
  * the number of stage launches and their summed duration in milliseconds
  * (reading synchronises and resets the counters). */
 int nemo_gpu_set_timing(void *handle, int32_t enable);
+/* Measurement aid: one plain device copy of nbytes (reads and writes exactly
+ * nbytes), the known-size kernel that scales hardware traffic counters. */
+int nemo_gpu_copy_calibration(void *handle, int64_t nbytes);
 int nemo_gpu_read_timing(void *handle, int64_t *launches, double *total_ms);
 
 /* ---- learned troubled-cell indicator --------------------------------------

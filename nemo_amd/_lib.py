@@ -89,6 +89,7 @@ SIGNATURES = {
     "nemo_gpu_batch_set_launch": [_VP, ctypes.c_int32, ctypes.c_int32],
     "nemo_gpu_batch_autotune": [_VP, ctypes.c_int32, _PI32, _PI32, _P],
     "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
+    "nemo_gpu_copy_calibration": [_VP, ctypes.c_int64],
     "nemo_gpu_read_timing": [_VP, _PI64, _P],
     "nemo_gpu_operators": [ctypes.c_int32, _P, _P, _P, _P],
     "nemo_gpu_error_message": [ctypes.c_char_p, ctypes.c_int32],

@@ -792,6 +792,35 @@ int nemo_gpu_mlp_train(void* handle, int64_t n, const float* features, const flo
     return NEMO_OK;
 }
 
+// Plain copy of n doubles (the known-size kernel that calibrates the HBM
+// traffic counters: it reads and writes exactly 8n bytes).
+__global__ void nemo_copy_kernel(const double* __restrict__ src, double* __restrict__ dst, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int nemo_gpu_copy_calibration(void* handle, int64_t nbytes) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || nbytes <= 0 || nbytes % 8) return fail(NEMO_ERR_ARG, "nbytes must be a positive multiple of 8");
+    double *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, size_t(nbytes)));
+    if (hipMalloc(&b, size_t(nbytes)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(NEMO_ERR_HIP, "hipMalloc failed");
+    }
+    const long long n = nbytes / 8;
+    hipError_t e = hipMemsetAsync(a, 0, size_t(nbytes), s->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(nemo_copy_kernel, dim3(4096), dim3(256), 0, s->stream, a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(NEMO_ERR_HIP, std::string("copy calibration: ") + hipGetErrorString(e));
+    return NEMO_OK;
+}
+
 int nemo_gpu_set_timing(void* handle, int32_t enable) {
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail(NEMO_ERR_ARG, "null handle");

@@ -512,7 +512,7 @@ struct StageLds {
     using TU = typename std::conditional<COMPACT, float, double>::type;
     static constexpr int OP_DOUBLES = (V * NN * int(sizeof(TU)) + V * NN * int(sizeof(Real)) + 7) / 8;
     static constexpr int RAW_DOUBLES = OP_DOUBLES > V * NN ? OP_DOUBLES : V * NN;
-    static constexpr int DOUBLES = RAW_DOUBLES + (COMPACT ? NN : 1);
+    static constexpr int DOUBLES = RAW_DOUBLES + (COMPACT ? NN : 0);
 };
 
 // One block of BE elements through a whole stage (the body of the stage

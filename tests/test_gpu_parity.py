@@ -164,3 +164,30 @@ def test_repair_parity():
     err = _rel(u, uref)
     print(f"  rel Linf {err:.3e}")
     assert err <= 1e-10
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_partial_last_block(N):
+    """125 elements (5^3) leave a partial last workgroup tile for every
+    degree (7, 4 and 2 elements per workgroup): RHS, indicator, CFL and a
+    20-step run still match the oracle."""
+    m, u0 = _setup("kh", 5, N)
+    g, o = _cfgs(N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, repair=1)
+    print(f"partial block N={N}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        r = s.kernel_calc_flux()
+        a = s.indicator()
+        dt, _, _ = s.hook_calc_timestep()
+        h = s.run(20, record_dt=True)
+        u = s.get_state()
+    aref, _ = O.indicator(o, u0)
+    assert np.abs(a - aref).max() <= 1e-12
+    assert _rel(r, O.rhs(o, m.nbr, m.delta, u0, aref)) <= RHS_TOL
+    assert abs(dt - O.calc_timestep(o, m.delta, u0)[0]) <= 1e-15 * dt
+    uref = u0.copy()
+    rc, href = O.run(o, m.nbr, m.delta, uref, 20)
+    assert rc == 0
+    assert np.abs(h - href).max() <= 1e-12 * href.max()
+    assert _rel(u, uref) <= RUN_TOL

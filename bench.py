@@ -29,6 +29,11 @@ Prints one JSON line; `value` is the plain FP64 path.  Extra objects:
   ensemble      the batched ensemble: 256 independent Riemann-problem runs
                 of uneven size and N through persistent, graph-captured stage
                 launches with an autotuned launch geometry.
+  amr_sedov     Sedov blast on an adaptive forest (levels 3-6) refined by the
+                indicator every 8 steps: FP64 + Persson-Peraire and the chosen
+                form (FP32 + learned indicator), each adapting on its own
+                factors, and the chosen form replayed on the FP64 run's meshes
+                and time steps for its L1 density error.
   cpu_baseline  the FP64 C oracle on the host cores (OpenMP), timed on a
                 bounded sample of the same workload.
 """
@@ -190,6 +195,73 @@ def ensemble(nruns: int, steps: int, seed: int):
             "setup_s": round(t_setup, 1)}
 
 
+def amr_sedov(steps: int, seed: int, lmin: int = 3, lmax: int = 6, every: int = 8):
+    """Sedov blast (E = 1 in a Gaussian of radius 0.02, rho = 1, p = 1e-5,
+    seeded centre jitter) on an adaptive forest of levels lmin..lmax, N = 3,
+    refinement driven by the blending factor of the indicator every `every`
+    steps.  Legs: (a) FP64 operator + Persson-Peraire indicator, adapting on
+    its own factors, recorded; (b) the chosen form (FP32 operator + learned
+    indicator) adapting on its own factors, timed; (c) the chosen form
+    replaying (a)'s meshes and time steps, for its L1 density error against
+    (a) on the same final mesh."""
+    from nemo_amd import _lib, amr, amr_run, indicator_mlp, operators, solver
+    N = 3
+    kw = dict(N=N, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC, timedisc=_lib.TD_LSRK54, repair=1, cfl=0.5,
+              alpha_max=0.5, alpha_min=0.001)
+    cfg64 = solver.make_config(indicator=_lib.IND_PP, precision=0, **kw)
+    cfg32 = solver.make_config(indicator=_lib.IND_MLP, precision=1, **kw)
+    params = indicator_mlp.load_params()
+    ic = lambda f: indicator_mlp.sedov(f, N, seed=seed, radius=0.02, p0=1e-5)
+    f0 = amr_run.initial_forest(lambda: solver.Solver(cfg64), amr.Forest.uniform(lmin, minlevel=lmin, maxlevel=lmax),
+                                ic)
+    u0 = ic(f0)
+    M3 = (N + 1) ** 3
+    uniform = (1 << lmax) ** 3
+    res = {"workload": f"Sedov blast E=1, Gaussian radius 0.02, p0=1e-5, levels {lmin}-{lmax}, N={N}, "
+                       f"adapt every {every} steps, seed {seed}",
+           "steps": steps, "initial_elements": f0.nelem, "uniform_elements": uniform,
+           "initial_levels": np.bincount(f0.level, minlength=lmax + 1)[lmin:].tolist()}
+    legs = {}
+    for name, cfg in (("fp64_pp", cfg64), ("chosen", cfg32)):
+        with solver.Solver(cfg) as s:
+            s.upload_mesh(f0)
+            s.set_state(u0)
+            if cfg is cfg32:
+                s.mlp_set_params(params)
+            run = amr_run.AdaptiveRun(s, f0, every=every)
+            t0 = time.perf_counter()
+            run.advance(steps, record=(name == "fp64_pp"))
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            leg = {"value": run.dof_updates / wall, "unit": "DOF-updates/s",
+                   "uniform_equivalent": uniform * M3 * NSTAGES * steps / wall,
+                   "ms_per_step": wall / steps * 1e3, "remeshes": run.remeshes, "fallbacks": run.fallbacks,
+                   "remesh_ms_total": round(run.remesh_seconds * 1e3, 1),
+                   "elements_min": int(min(run.elements)), "elements_max": int(max(run.elements)),
+                   "final_elements": run.forest.nelem,
+                   "final_levels": np.bincount(run.forest.level, minlength=lmax + 1)[lmin:].tolist()}
+            if cfg is cfg32:
+                j, o, fb = s.mlp_stats()
+                leg["judge"] = {"judged": j, "overridden": o, "fallback": fb}
+            else:
+                history, f_end, u_end = run.history, run.forest, s.get_state()
+            legs[name] = leg
+    with solver.Solver(cfg32) as s:
+        s.upload_mesh(f0)
+        s.set_state(u0)
+        s.mlp_set_params(params)
+        amr_run.replay(s, history)
+        u_rep = s.get_state()
+    _, w = operators.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    vol = np.prod(f_end.delta, 1)[:, None] * w3[None, :] / 8.0
+    err = float(np.sum(np.abs(u_rep[:, 0] - u_end[:, 0]) * vol) / np.sum(np.abs(u_end[:, 0]) * vol))
+    res.update(legs)
+    res["chosen_vs_fp64_same_meshes"] = {"l1_density_err": err, "l1_bar": 1e-6, "meets_l1_bar": err <= 1e-6,
+                                         "final_elements": f_end.nelem}
+    return res
+
+
 def cpu_baseline(n: int, N: int, seed: int, steps: int, cfg_kw):
     """FP64 oracle on the host cores: one or more full steps of the same
     64^3 workload, after one untimed warm-up stage sequence on a small mesh."""
@@ -228,6 +300,8 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--amr-steps", type=int, default=200)
+    ap.add_argument("--no-amr", action="store_true", help="skip the adaptive Sedov leg")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one plain device copy of 1 GiB: the known-size kernel that scales "
                          "FETCH_SIZE/WRITE_SIZE in counter runs")
@@ -318,6 +392,8 @@ def main():
         out["precision_mhd"] = mhd_precision(args.mhd_elems, args.mhd_steps, args.seed)
     if not args.no_ensemble:
         out["ensemble"] = ensemble(args.ensemble_runs, args.ensemble_steps, args.seed)
+    if not args.no_amr:
+        out["amr_sedov"] = amr_sedov(args.amr_steps, args.seed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.elems, N, args.seed, args.cpu_steps, cfg_kw)
     print(json.dumps(out), flush=True)

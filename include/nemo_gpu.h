@@ -91,6 +91,31 @@ int nemo_gpu_destroy(void *handle);
  * (quad%delta).  Conforming, aligned faces. */
 int nemo_gpu_upload_mesh(void *handle, int64_t nelem, const int32_t *nbr, const double *delta);
 
+/* Adaptive (2:1 balanced) mesh: as nemo_gpu_upload_mesh plus the sides with
+ * nquads = 4 of mesh_alloc (mesh/p4est/mesh_utils_mod.f90:340-376, side_t in
+ * mesh/p4est/mesh_types_mod.f90:56-78).  mortars[m] = (big element, its face,
+ * small_0..small_3); small_k covers half (k & 1) of the big face along the
+ * face's first tangential axis and half (k >> 1) along the second (axes in
+ * increasing order).  Faces on mortar m carry nbr = -(5m+1) (big) and
+ * -(5m+2+k) (small_k).  Their fluxes follow the reference's rfaceMat /
+ * cfaceMat sequence (kernel/dgsem/weakform/2d/kernel_flux_mod.f90:58-147).
+ * Euler only. */
+int nemo_gpu_upload_mesh_amr(void *handle, int64_t nelem, const int32_t *nbr, const double *delta,
+                             int64_t nmortar, const int32_t *mortars);
+
+/* Mesh adaptation: replaces the state-transfer half of mesh_adapt_wrapped
+ * (mesh/p4est/mesh_utils_mod.f90:966-1000) with kernel_amr_refine /
+ * kernel_amr_coarsen (kernel/mcfv/3d/kernel_amr_mod.f90:12-70).  The new
+ * mesh is given as for nemo_gpu_upload_mesh_amr; units[u] = (kind, first old
+ * element, first new element) in element order, kind 0 = copy, 1 = refine
+ * (one old into the 8 new children, child c = x + 2y + 4z), 2 = coarsen (8
+ * old siblings into one).  A unit whose interpolated or projected state has
+ * a nonpositive density or pressure takes the (conserved) element mean
+ * instead; their number is returned in *fallbacks.  RK registers are
+ * cleared: call between time steps.  Euler only. */
+int nemo_gpu_remesh(void *handle, int64_t nelem, const int32_t *nbr, const double *delta, int64_t nmortar,
+                    const int32_t *mortars, int64_t nunits, const int32_t *units, int64_t *fallbacks);
+
 /* State upload/download (payload quad%pld%hydro%state of every quad). */
 int nemo_gpu_upload_state(void *handle, const double *u);
 int nemo_gpu_download_state(void *handle, double *u);

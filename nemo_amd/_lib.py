@@ -60,6 +60,9 @@ SIGNATURES = {
     "nemo_gpu_create": [ctypes.POINTER(Config), ctypes.POINTER(_VP)],
     "nemo_gpu_destroy": [_VP],
     "nemo_gpu_upload_mesh": [_VP, ctypes.c_int64, _PI32, _P],
+    "nemo_gpu_upload_mesh_amr": [_VP, ctypes.c_int64, _PI32, _P, ctypes.c_int64, _PI32],
+    "nemo_gpu_remesh": [_VP, ctypes.c_int64, _PI32, _P, ctypes.c_int64, _PI32, ctypes.c_int64, _PI32,
+                        ctypes.POINTER(ctypes.c_int64)],
     "nemo_gpu_upload_state": [_VP, _P],
     "nemo_gpu_download_state": [_VP, _P],
     "nemo_gpu_rhs": [_VP, _P],

@@ -1,0 +1,371 @@
+"""Host-side adaptive octree mesh: 2:1-balanced leaves of one periodic tree.
+
+Stands in for the p4est forest of the reference (mesh/p4est/mesh_utils_mod.f90)
+on the configurations it runs: a single periodic cube tree refined to levels
+between ``minlevel`` and ``maxlevel``.  Leaves are kept in Morton (Z-order)
+order of their anchors, which is the order p4est gives the leaves of one tree,
+so families (the 8 children of one parent) are consecutive with child 0 first
+and child c = x + 2y + 4z.
+
+Connectivity follows the reference's side model (quad_t.isides, side_t with
+nquads = 1 or 4, mesh/p4est/mesh_types_mod.f90:30-100, built in mesh_alloc,
+mesh_utils_mod.f90:340-376), flattened into the device upload format:
+
+    nbr[e, f] >= 0        conforming neighbour element across face f
+    nbr[e, f] = -(s + 1)  face f of e lies on a non-conforming (2:1) interface;
+                          its flux is read from face slot s
+    mortars[m] = (big, face, small_0, small_1, small_2, small_3)
+
+Mortar m owns face slots 5m (the big element's face) and 5m + 1 + k (the face
+of small_k).  small_k covers the half k & 1 of the big face along its first
+tangential axis and the half k >> 1 along its second one (tangential axes in
+increasing order: x-faces (y, z), y-faces (x, z), z-faces (x, y)); this is the
+face-child order of the reference's 3-D face arrays
+(kernel/mcfv/3d/kernel_flux_mod.f90:100-121, rfaceMat(:,:,iquad) on the first
+and rfaceTam(:,:,iquad) on the second face index).
+
+Balance is 2:1 across faces only: DG couples elements through faces, so edge
+and corner neighbours may differ by more than one level.
+
+Adaptation (``adapt``) follows mesh_adapt_wrapped (mesh_utils_mod.f90:879-1009):
+flag leaves for refinement, flag whole families for coarsening, coarsen, refine,
+balance, then pair old and new leaves in Morton order into replacement units
+(copy, refine one into eight, coarsen eight into one) for the state transfer.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mesh import morton_key
+
+GRID_BITS = 16           # anchors are integers on a 2^16 lattice per direction
+ROOT = 1 << GRID_BITS
+
+UNIT_COPY, UNIT_REFINE, UNIT_COARSEN = 0, 1, 2
+
+# tangential axes of a face normal to axis d, in increasing order
+TANGENTIAL = ((1, 2), (0, 2), (0, 1))
+
+
+def _keys(level: np.ndarray, anchor: np.ndarray) -> np.ndarray:
+    """Lookup key of (level, anchor): Morton code of the anchor, then level."""
+    m = morton_key(anchor[:, 0], anchor[:, 1], anchor[:, 2]).astype(np.int64)
+    return m * 32 + level.astype(np.int64)
+
+
+class _Index:
+    """Sorted-key lookup of leaves by (level, anchor)."""
+
+    def __init__(self, level: np.ndarray, anchor: np.ndarray):
+        k = _keys(level, anchor)
+        self.order = np.argsort(k, kind="stable")
+        self.sorted = k[self.order]
+
+    def find(self, level: np.ndarray, anchor: np.ndarray) -> np.ndarray:
+        """Leaf index of each (level, anchor), or -1."""
+        k = _keys(level, anchor)
+        pos = np.searchsorted(self.sorted, k)
+        pos = np.minimum(pos, len(self.sorted) - 1)
+        hit = self.sorted[pos] == k
+        return np.where(hit, self.order[pos], -1)
+
+
+def _size(level: np.ndarray) -> np.ndarray:
+    return np.left_shift(np.int64(1), (GRID_BITS - level.astype(np.int64)))
+
+
+def _morton_sort(level: np.ndarray, anchor: np.ndarray):
+    key = morton_key(anchor[:, 0], anchor[:, 1], anchor[:, 2])
+    p = np.argsort(key, kind="stable")
+    return level[p], anchor[p]
+
+
+def _children(level: np.ndarray, anchor: np.ndarray):
+    """The 8 children of each leaf, child c = x + 2y + 4z, grouped per parent."""
+    h = _size(level + 1)
+    c = np.arange(8)
+    off = np.stack([c & 1, (c >> 1) & 1, (c >> 2) & 1], axis=1)             # [8, 3]
+    ach = anchor[:, None, :] + off[None, :, :] * h[:, None, None]          # [n, 8, 3]
+    lch = np.repeat(level + 1, 8)
+    return lch, ach.reshape(-1, 3)
+
+
+def _containing_level(index: _Index, level_of: np.ndarray, point: np.ndarray, lmin: int, lmax: int) -> np.ndarray:
+    """Level of the leaf that contains each lattice point (-1 if none found)."""
+    out = np.full(len(point), -1, dtype=np.int64)
+    todo = np.ones(len(point), dtype=bool)
+    for lv in range(lmax, lmin - 1, -1):
+        if not todo.any():
+            break
+        mask = ~(np.int64(_size(np.array([lv]))[0]) - 1)
+        idx = np.nonzero(todo)[0]
+        a = point[idx] & mask
+        hit = index.find(np.full(len(idx), lv), a)
+        ok = hit >= 0
+        out[idx[ok]] = lv
+        todo[idx[ok]] = False
+    return out
+
+
+@dataclass
+class Forest:
+    """Leaves of one periodic tree over the box [origin, origin + extent]."""
+    level: np.ndarray           # int64 [n]
+    anchor: np.ndarray          # int64 [n, 3], lattice units (2^16 per box edge)
+    minlevel: int
+    maxlevel: int
+    extent: tuple = (1.0, 1.0, 1.0)
+    origin: tuple = (0.0, 0.0, 0.0)
+    _conn: tuple = field(default=None, repr=False, compare=False)
+
+    # ---- construction ----------------------------------------------------
+    @classmethod
+    def uniform(cls, level: int, minlevel: int = None, maxlevel: int = None, extent=(1.0, 1.0, 1.0),
+                origin=(0.0, 0.0, 0.0)) -> "Forest":
+        n = 1 << level
+        ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        h = ROOT >> level
+        anchor = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.int64) * h
+        lv, an = _morton_sort(np.full(n ** 3, level, dtype=np.int64), anchor)
+        return cls(lv, an, level if minlevel is None else minlevel, level if maxlevel is None else maxlevel,
+                   tuple(extent), tuple(origin))
+
+    @property
+    def nelem(self) -> int:
+        return int(self.level.shape[0])
+
+    def _replace(self, level, anchor) -> "Forest":
+        lv, an = _morton_sort(level, anchor)
+        return Forest(lv, an, self.minlevel, self.maxlevel, self.extent, self.origin)
+
+    def refine(self, flags: np.ndarray) -> "Forest":
+        """Replace every flagged leaf by its 8 children (no balancing)."""
+        flags = np.asarray(flags, dtype=bool)
+        lch, ach = _children(self.level[flags], self.anchor[flags])
+        level = np.concatenate([self.level[~flags], lch])
+        anchor = np.concatenate([self.anchor[~flags], ach])
+        return self._replace(level, anchor)
+
+    def family_starts(self) -> np.ndarray:
+        """Boolean mask: leaf i is child 0 of a complete family i..i+7."""
+        n = self.nelem
+        out = np.zeros(n, dtype=bool)
+        if n < 8:
+            return out
+        lv = self.level
+        cand = np.nonzero(lv[: n - 7] > 0)[0]
+        h = _size(lv[cand])
+        first = np.all((self.anchor[cand] & (2 * h)[:, None] - 1) == 0, axis=1)
+        cand = cand[first]
+        ok = np.ones(len(cand), dtype=bool)
+        base = self.anchor[cand]
+        hh = _size(lv[cand])
+        for c in range(8):
+            off = np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], dtype=np.int64)
+            j = cand + c
+            ok &= lv[j] == lv[cand]
+            ok &= np.all(self.anchor[j] == base + off[None, :] * hh[:, None], axis=1)
+        out[cand[ok]] = True
+        return out
+
+    def coarsen(self, starts: np.ndarray) -> "Forest":
+        """Replace each family starting at a True entry of ``starts`` by its parent."""
+        starts = np.asarray(starts, dtype=bool)
+        if not starts.any():
+            return self
+        drop = np.zeros(self.nelem, dtype=bool)
+        idx = np.nonzero(starts)[0]
+        for c in range(8):
+            drop[idx + c] = True
+        level = np.concatenate([self.level[~drop], self.level[idx] - 1])
+        anchor = np.concatenate([self.anchor[~drop], self.anchor[idx]])
+        return self._replace(level, anchor)
+
+    def balance(self) -> "Forest":
+        """Refine until face neighbours differ by at most one level (2:1)."""
+        f = self
+        while True:
+            index = _Index(f.level, f.anchor)
+            h = _size(f.level)
+            need = np.zeros(f.nelem, dtype=bool)
+            lmin, lmax = int(f.level.min()), int(f.level.max())
+            for d in range(3):
+                for s in (-1, 1):
+                    p = f.anchor.copy()
+                    p[:, d] = (p[:, d] + s * h) % ROOT
+                    lv = _containing_level(index, f.level, p, lmin, lmax)
+                    bad = (lv >= 0) & (lv < f.level - 1)
+                    if bad.any():
+                        mask = ~(_size(lv[bad]) - 1)
+                        a = p[bad] & mask[:, None]
+                        hit = index.find(lv[bad], a)
+                        need[hit[hit >= 0]] = True
+            if not need.any():
+                return f
+            f = f.refine(need)
+
+    # ---- geometry and connectivity ---------------------------------------
+    @property
+    def delta(self) -> np.ndarray:
+        h = _size(self.level).astype(np.float64) / ROOT
+        return np.ascontiguousarray(h[:, None] * np.asarray(self.extent)[None, :])
+
+    @property
+    def lower(self) -> np.ndarray:
+        return np.ascontiguousarray(np.asarray(self.origin)[None, :]
+                                    + self.anchor.astype(np.float64) / ROOT * np.asarray(self.extent)[None, :])
+
+    def node_coords(self, xi: np.ndarray) -> np.ndarray:
+        """Physical coordinates [nelem, 3, M^3] of the LGL nodes xi (on [-1,1])."""
+        M = len(xi)
+        ii, jj, kk = np.meshgrid(np.arange(M), np.arange(M), np.arange(M), indexing="ij")
+        ref = np.stack([xi[ii].ravel(order="F"), xi[jj].ravel(order="F"), xi[kk].ravel(order="F")])
+        return self.lower[:, :, None] + 0.5 * (ref[None, :, :] + 1.0) * self.delta[:, :, None]
+
+    def connectivity(self):
+        """(nbr [n, 6] int32, mortars [nm, 6] int32) of the upload format."""
+        if self._conn is not None:
+            return self._conn
+        n = self.nelem
+        index = _Index(self.level, self.anchor)
+        h = _size(self.level)
+        nbr = np.full((n, 6), np.iinfo(np.int32).min, dtype=np.int64)
+        big_faces = []  # (element, face, children[4])
+        for d in range(3):
+            t1, t2 = TANGENTIAL[d]
+            for side in (0, 1):
+                f = 2 * d + side
+                s = 1 if side else -1
+                p = self.anchor.copy()
+                p[:, d] = (p[:, d] + s * h) % ROOT
+                same = index.find(self.level, p)
+                nbr[same >= 0, f] = same[same >= 0]
+                rest = np.nonzero(same < 0)[0]
+                if len(rest) == 0:
+                    continue
+                # coarser neighbour: this element is a small side (filled from the big side below)
+                lvl = self.level[rest]
+                pm = p[rest] & ~(2 * h[rest] - 1)[:, None]
+                coarse = index.find(lvl - 1, pm)
+                fine_idx = rest[coarse < 0]
+                if len(fine_idx) == 0:
+                    continue
+                # finer neighbours: this element is the big side of a mortar
+                hh = h[fine_idx] // 2
+                kids = np.empty((len(fine_idx), 4), dtype=np.int64)
+                for k in range(4):
+                    a = p[fine_idx].copy()
+                    if side == 0:
+                        a[:, d] += hh
+                    a[:, t1] += (k & 1) * hh
+                    a[:, t2] += (k >> 1) * hh
+                    kids[:, k] = index.find(self.level[fine_idx] + 1, a)
+                if (kids < 0).any():
+                    raise ValueError("mesh is not 2:1 balanced across faces")
+                big_faces.append(np.concatenate([fine_idx[:, None], np.full((len(fine_idx), 1), f), kids], axis=1))
+        if big_faces:
+            mortars = np.concatenate(big_faces).astype(np.int64)
+            mortars = mortars[np.lexsort((mortars[:, 1], mortars[:, 0]))]
+        else:
+            mortars = np.zeros((0, 6), dtype=np.int64)
+        m = np.arange(len(mortars))
+        nbr[mortars[:, 0], mortars[:, 1]] = -(5 * m + 1)
+        fo = mortars[:, 1] ^ 1
+        for k in range(4):
+            nbr[mortars[:, 2 + k], fo] = -(5 * m + 2 + k)
+        mortars = np.ascontiguousarray(mortars.astype(np.int32))
+        if (nbr == np.iinfo(np.int32).min).any():
+            raise ValueError("unresolved face (mesh is not 2:1 balanced)")
+        self._conn = (np.ascontiguousarray(nbr.astype(np.int32)), mortars)
+        return self._conn
+
+    @property
+    def nbr(self) -> np.ndarray:
+        return self.connectivity()[0]
+
+    @property
+    def mortars(self) -> np.ndarray:
+        return self.connectivity()[1]
+
+    # ---- adaptation --------------------------------------------------------
+    def adapt(self, refine: np.ndarray, coarsen: np.ndarray):
+        """One adaptation pass (mesh_utils_mod.f90:879-1009).
+
+        ``refine[i]``: leaf i wants refinement (honoured below maxlevel).
+        ``coarsen[i]``: leaf i may be merged with its siblings; a family is
+        coarsened when all 8 members agree, none wants refinement and the
+        family is above minlevel.  Returns (new forest, units) where units is
+        int32 [nunits, 3] = (kind, first old leaf, first new leaf), kinds
+        UNIT_COPY (1 -> 1), UNIT_REFINE (1 -> 8 children in child order) and
+        UNIT_COARSEN (8 siblings -> 1), in Morton order; or (self, None) if
+        nothing changes.
+        """
+        refine = np.asarray(refine, dtype=bool) & (self.level < self.maxlevel)
+        coarsen = np.asarray(coarsen, dtype=bool) & ~refine
+        starts = self.family_starts() & (self.level > self.minlevel)
+        idx = np.nonzero(starts)[0]
+        ok = np.ones(len(idx), dtype=bool)
+        for c in range(8):
+            ok &= coarsen[idx + c]
+        starts[:] = False
+        starts[idx[ok]] = True
+        if not refine.any() and not starts.any():
+            return self, None
+        new = self.coarsen_refine(starts, refine)
+        new = new.balance()
+        if new.nelem == self.nelem and np.array_equal(new.level, self.level) and np.array_equal(new.anchor,
+                                                                                                 self.anchor):
+            return self, None
+        return new, replacement_units(self, new)
+
+    def coarsen_refine(self, starts: np.ndarray, refine: np.ndarray) -> "Forest":
+        """Coarsen the flagged families, then refine the flagged leaves."""
+        drop = np.zeros(self.nelem, dtype=bool)
+        idx = np.nonzero(starts)[0]
+        for c in range(8):
+            drop[idx + c] = True
+        keep = ~drop & ~refine
+        lch, ach = _children(self.level[refine & ~drop], self.anchor[refine & ~drop])
+        level = np.concatenate([self.level[keep], self.level[idx] - 1, lch])
+        anchor = np.concatenate([self.anchor[keep], self.anchor[idx], ach])
+        return self._replace(level, anchor)
+
+
+def replacement_units(old: Forest, new: Forest) -> np.ndarray:
+    """Pair old and new leaves (both in Morton order) into transfer units.
+
+    Walks both leaf sequences in lockstep like the replace loop of
+    mesh_adapt_wrapped (mesh_utils_mod.f90:967-1000): a new leaf one level
+    finer than the old one starts a refine unit (8 new), one level coarser a
+    coarsen unit (8 old), equal level and anchor a copy.
+    """
+    units = []
+    i = j = 0
+    no, nn = old.nelem, new.nelem
+    while i < no and j < nn:
+        lo, ln = old.level[i], new.level[j]
+        if lo == ln:
+            if not np.array_equal(old.anchor[i], new.anchor[j]):
+                raise ValueError("old and new leaves out of step")
+            units.append((UNIT_COPY, i, j))
+            i += 1
+            j += 1
+        elif ln == lo + 1:
+            if not (np.all(new.level[j:j + 8] == ln) and np.array_equal(new.anchor[j], old.anchor[i])):
+                raise ValueError("refined leaf without its 8 children")
+            units.append((UNIT_REFINE, i, j))
+            i += 1
+            j += 8
+        elif ln == lo - 1:
+            if not (np.all(old.level[i:i + 8] == lo) and np.array_equal(new.anchor[j], old.anchor[i])):
+                raise ValueError("coarsened leaf without its 8 children")
+            units.append((UNIT_COARSEN, i, j))
+            i += 8
+            j += 1
+        else:
+            raise ValueError("a leaf changed by more than one level")
+    if i != no or j != nn:
+        raise ValueError("old and new leaves do not cover the same domain")
+    return np.asarray(units, dtype=np.int32).reshape(-1, 3)

@@ -74,7 +74,8 @@ struct StageArgs {
     double* __restrict__ r1;
     double* __restrict__ rhs_out;    // non-null: write the RHS, no update
     double* __restrict__ alpha_out;  // optional per-element blending factor
-    const int32_t* __restrict__ nbr; // [nelem][6]
+    const int32_t* __restrict__ nbr; // [nelem][6]; -(s+1): face flux from mortar_flux slot s
+    const double* __restrict__ mortar_flux;  // [slot][var][M*M] fluxes of the non-conforming faces
     const double* __restrict__ delta;// [nelem][3]
     const double* __restrict__ dt_dev;
     const double* __restrict__ glm;  // MHD: [c_h, alpha] of the current step

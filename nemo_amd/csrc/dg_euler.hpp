@@ -49,14 +49,21 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     Real fm[5], fp[5];
     {
         NodeState<Real> Nm, Np;
+        // faces on a non-conforming interface take their flux from the
+        // mortar pass; the Riemann solve below then runs on the element's
+        // own face node and is discarded
+        const double* mort_m = nullptr;
+        const double* mort_p = nullptr;
         if (valid) {
             const long long em = a.nbr[e * 6 + 2 * D];
             const long long ep = a.nbr[e * 6 + 2 * D + 1];
             const double dx = a.delta[e * 3 + D];
             jac = Real(2.0 / dx);
             sdx = Real(double(M) / dx);
-            const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
-            const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
+            if (em < 0) mort_m = a.mortar_flux + size_t(-em - 1) * 5 * M2 + ta + M * tb;
+            if (ep < 0) mort_p = a.mortar_flux + size_t(-ep - 1) * 5 * M2 + ta + M * tb;
+            const double* um = a.u_in + (em < 0 ? e : em) * M3 + nidx<D, M>(em < 0 ? 0 : M - 1, ta, tb);
+            const double* up = a.u_in + (ep < 0 ? e : ep) * M3 + nidx<D, M>(ep < 0 ? M - 1 : 0, ta, tb);
             double dm[5], dp[5];
 #pragma unroll
             for (int h = 0; h < 5; ++h) {
@@ -82,6 +89,12 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         make_state<D, false>(g, Np);
         riemann<SURF, D>(g, Nm, S[0], fm);
         riemann<SURF, D>(g, S[M - 1], Np, fp);
+        if (mort_m)
+#pragma unroll
+            for (int h = 0; h < 5; ++h) fm[h] = Real(mort_m[h * M2]);
+        if (mort_p)
+#pragma unroll
+            for (int h = 0; h < 5; ++h) fp[h] = Real(mort_p[h * M2]);
     }
 
     Real c[M][5];

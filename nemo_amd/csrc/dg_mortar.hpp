@@ -1,0 +1,292 @@
+// Non-conforming (2:1) faces and the state transfer of mesh adaptation
+// (gfx950).
+//
+// mortar_kernel: the reference's mortar sequence of
+// kernel_fill_faceflux_cb (kernel/dgsem/weakform/2d/kernel_flux_mod.f90:
+// 58-147, 3-D face arrays as in kernel/mcfv/3d/kernel_flux_mod.f90:100-121):
+// the big element's face values are interpolated to the nodes of each of the
+// four half-size faces (rfaceMat), the Riemann problem is solved at those
+// nodes, each small element takes its fluxes as they are and the big element
+// takes their projection (cfaceMat) summed over the four.  One workgroup per
+// mortar, one lane per small-face node; the face fluxes go to slots of a
+// small buffer that the stage kernel reads in place of its own Riemann solve
+// on those faces.
+//
+// remap_kernel: kernel_amr_refine / kernel_amr_coarsen
+// (kernel/mcfv/3d/kernel_amr_mod.f90:12-70) with the positivity fallback of
+// setup_amr_replace_refine/coarsen (setups/euler/3d/sedov/setup_amr_mod.f90:
+// 76-130): one workgroup per replacement unit (copy, one parent into eight
+// children, eight children into one parent), one lane per node.
+#pragma once
+
+#include "dg_euler.hpp"
+
+namespace nemo {
+
+template <int M>
+struct MortarOps {
+    double P[2][M][M];  // P[c][i][p]: parent node p -> node i of half c
+    double R[2][M][M];  // R[c][p][i]: node i of half c -> parent node p (projection)
+};
+
+template <int M>
+struct MortarShape {
+    static constexpr int M2 = M * M;
+    static constexpr int NT = (4 * M2 + 63) / 64 * 64;
+};
+
+template <int M, typename Real>
+struct MortarArgs {
+    const double* __restrict__ u_in;     // [var][elem][node]
+    double* __restrict__ flux;           // [slot][var][M*M]
+    const int32_t* __restrict__ mortars; // [nmortar][6]: big, face, small_0..3
+    long long nelem;
+    Gas<Real> gas;
+    Phys64 ph64;
+    MortarOps<M> op;
+};
+
+// Node index of face node (ta, tb) at line position l of direction d.
+template <int M>
+__device__ __forceinline__ int face_node(int d, int l, int ta, int tb) {
+    return d == 0 ? nidx<0, M>(l, ta, tb) : (d == 1 ? nidx<1, M>(l, ta, tb) : nidx<2, M>(l, ta, tb));
+}
+
+template <int SURF, int D, typename Real>
+__device__ __forceinline__ void mortar_riemann_d(const Gas<Real>& g, const Phys64& P, const double* uL,
+                                                 const double* uR, Real* f) {
+    NodeState<Real> L, R;
+#pragma unroll
+    for (int h = 0; h < 5; ++h) {
+        L.u[h] = Real(uL[h]);
+        R.u[h] = Real(uR[h]);
+    }
+    if constexpr (sizeof(Real) == 4) {
+        L.p64 = pressure(P.gas, uL);
+        R.p64 = pressure(P.gas, uR);
+    }
+    make_state<D, false>(g, L);
+    make_state<D, false>(g, R);
+    riemann<SURF, D>(g, L, R, f);
+}
+
+template <int M, int SURF, typename Real>
+__global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M, Real> a) {
+    constexpr int M2 = M * M, M3 = M2 * M;
+    __shared__ double s_big[5][M2];
+    __shared__ double s_f[5][4 * M2];
+    const int tid = threadIdx.x;
+    const long long m = blockIdx.x;
+    const int32_t* mt = a.mortars + m * 6;
+    const long long big = mt[0];
+    const int fb = mt[1];
+    const int d = fb >> 1;
+    const bool big_left = (fb & 1) != 0;  // big face is x+/y+/z+: big state is u_L
+    const long long EM3 = a.nelem * (long long)M3;
+    if (tid < M2) {
+        const int n = face_node<M>(d, big_left ? M - 1 : 0, tid % M, tid / M);
+#pragma unroll
+        for (int h = 0; h < 5; ++h) s_big[h][tid] = a.u_in[h * EM3 + big * M3 + n];
+    }
+    __syncthreads();
+    if (tid < 4 * M2) {
+        const int k = tid / M2, t = tid - k * M2;
+        const int ta = t % M, tb = t / M, ka = k & 1, kb = k >> 1;
+        const long long sm = mt[2 + k];
+        const int n = face_node<M>(d, big_left ? 0 : M - 1, ta, tb);
+        double us[5], ub[5];
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            us[h] = a.u_in[h * EM3 + sm * M3 + n];
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                double r = 0.0;
+#pragma unroll
+                for (int p = 0; p < M; ++p) r += a.op.P[ka][ta][p] * s_big[h][p + M * q];
+                acc += a.op.P[kb][tb][q] * r;
+            }
+            ub[h] = acc;
+        }
+        const double* uL = big_left ? ub : us;
+        const double* uR = big_left ? us : ub;
+        Real f[5];
+        if (d == 0)
+            mortar_riemann_d<SURF, 0>(a.gas, a.ph64, uL, uR, f);
+        else if (d == 1)
+            mortar_riemann_d<SURF, 1>(a.gas, a.ph64, uL, uR, f);
+        else
+            mortar_riemann_d<SURF, 2>(a.gas, a.ph64, uL, uR, f);
+        double* out = a.flux + (size_t(5 * m + 1 + k) * 5) * M2 + t;
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            out[h * M2] = double(f[h]);
+            s_f[h][tid] = double(f[h]);
+        }
+    }
+    __syncthreads();
+    if (tid < M2) {
+        const int p = tid % M, q = tid / M;
+        double* out = a.flux + (size_t(5 * m) * 5) * M2 + tid;
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int ka = k & 1, kb = k >> 1;
+#pragma unroll
+                for (int tb = 0; tb < M; ++tb) {
+                    double r = 0.0;
+#pragma unroll
+                    for (int ta = 0; ta < M; ++ta) r += a.op.R[ka][p][ta] * s_f[h][k * M2 + ta + M * tb];
+                    acc += a.op.R[kb][q][tb] * r;
+                }
+            }
+            out[h * M2] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// State transfer of one adaptation (Euler, FP64)
+// ---------------------------------------------------------------------------
+
+enum { UNIT_COPY = 0, UNIT_REFINE = 1, UNIT_COARSEN = 2 };
+
+template <int M>
+struct RemapShape {
+    static constexpr int M3 = M * M * M;
+    static constexpr int NT = (M3 + 63) / 64 * 64;
+};
+
+template <int M>
+struct RemapArgs {
+    const double* __restrict__ u_old;    // [var][old elem][node]
+    double* __restrict__ u_new;          // [var][new elem][node]
+    const int32_t* __restrict__ units;   // [nunits][3]: kind, first old, first new
+    long long nold, nnew;
+    Phys64 ph64;
+    MortarOps<M> op;
+    double w[M];
+    int* __restrict__ fallbacks;         // count of units that fell back to the mean
+};
+
+// Block sum of v over the NT lanes (result valid in every lane).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* s_red) {
+    const int tid = threadIdx.x;
+    s_red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (tid < off) s_red[tid] += s_red[tid + off];
+        __syncthreads();
+    }
+    const double r = s_red[0];
+    __syncthreads();
+    return r;
+}
+
+template <int M>
+__global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a) {
+    constexpr int M3 = M * M * M, NT = RemapShape<M>::NT;
+    __shared__ double s_in[8][5][M3];
+    __shared__ double s_red[NT];
+    const int tid = threadIdx.x;
+    const int32_t* ut = a.units + blockIdx.x * 3;
+    const int kind = ut[0];
+    const long long io = ut[1], jn = ut[2];
+    const long long OM3 = a.nold * (long long)M3, NM3 = a.nnew * (long long)M3;
+    const bool act = tid < M3;
+    const int i = tid % M, j = (tid / M) % M, k = tid / (M * M);
+    const double wn = act ? 0.125 * a.w[i] * a.w[j] * a.w[k] : 0.0;
+
+    if (kind == UNIT_COPY) {
+        if (act)
+#pragma unroll
+            for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + jn * M3 + tid] = a.u_old[h * OM3 + io * M3 + tid];
+        return;
+    }
+    const int nin = kind == UNIT_REFINE ? 1 : 8;
+    for (int c = 0; c < nin; ++c)
+        if (act)
+#pragma unroll
+            for (int h = 0; h < 5; ++h) s_in[c][h][tid] = a.u_old[h * OM3 + (io + c) * M3 + tid];
+    __syncthreads();
+
+    if (kind == UNIT_REFINE) {
+        // child c = cx + 2 cy + 4 cz at new element jn + c
+        double out[8][5];
+        bool ok = true;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+#pragma unroll
+            for (int h = 0; h < 5; ++h) {
+                double acc = 0.0;
+                if (act)
+                    for (int kk = 0; kk < M; ++kk) {
+                        double sj = 0.0;
+                        for (int jj = 0; jj < M; ++jj) {
+                            double si = 0.0;
+#pragma unroll
+                            for (int ii = 0; ii < M; ++ii)
+                                si += a.op.P[cx][i][ii] * s_in[0][h][ii + M * jj + M * M * kk];
+                            sj += a.op.P[cy][j][jj] * si;
+                        }
+                        acc += a.op.P[cz][k][kk] * sj;
+                    }
+                out[c][h] = acc;
+            }
+            if (act) ok = ok && isvalid64<EQ_EULER>(a.ph64, out[c]);
+        }
+        const bool bad = __syncthreads_or(!ok);
+        double mean[5];
+        if (bad) {
+#pragma unroll
+            for (int h = 0; h < 5; ++h) mean[h] = block_sum<NT>(act ? wn * s_in[0][h][tid] : 0.0, s_red);
+            if (tid == 0) atomicAdd(a.fallbacks, 1);
+        }
+        if (act)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + (jn + c) * M3 + tid] = bad ? mean[h] : out[c][h];
+        return;
+    }
+
+    // UNIT_COARSEN: children io .. io+7 (child order) into new element jn
+    double out[5];
+#pragma unroll
+    for (int h = 0; h < 5; ++h) {
+        double acc = 0.0;
+        if (act)
+            for (int c = 0; c < 8; ++c) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                for (int kk = 0; kk < M; ++kk) {
+                    double sj = 0.0;
+                    for (int jj = 0; jj < M; ++jj) {
+                        double si = 0.0;
+#pragma unroll
+                        for (int ii = 0; ii < M; ++ii)
+                            si += a.op.R[cx][i][ii] * s_in[c][h][ii + M * jj + M * M * kk];
+                        sj += a.op.R[cy][j][jj] * si;
+                    }
+                    acc += a.op.R[cz][k][kk] * sj;
+                }
+            }
+        out[h] = acc;
+    }
+    const bool ok = !act || isvalid64<EQ_EULER>(a.ph64, out);
+    const bool bad = __syncthreads_or(!ok);
+    if (bad) {
+#pragma unroll
+        for (int h = 0; h < 5; ++h) out[h] = block_sum<NT>(act ? wn * out[h] : 0.0, s_red);
+        if (tid == 0) atomicAdd(a.fallbacks, 1);
+    }
+    if (act)
+#pragma unroll
+        for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + jn * M3 + tid] = out[h];
+}
+
+}  // namespace nemo

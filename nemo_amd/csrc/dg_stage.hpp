@@ -39,14 +39,14 @@ namespace nemo {
 // normalised-Legendre basis (utils/interpol_mod.f90:274-311, 584-601): the
 // sum-factorised transform x -> y -> z, after which the calling thread holds
 // its line's partial sums of the total energy (tot), of the modes with all
-// indices below M-1 (c1) and of those with all indices below M-2 (c2), and
-// the energies of the top shell (some index M-1: s1 = tot - c1) and of the
-// top two shells (some index >= M-2: s2 = tot - c2) summed directly, free
-// of the cancellation in tot - c.
-// q must be complete (barrier) on entry; t1 and t2 are work rows.
+// indices below M-1 (c1), of the top shell (largest index M-1: s1) and of
+// the second shell (largest index M-2: sh2).  Every sum is direct, free of
+// the cancellation in tot - c1 or c1 - c2 when the high modes are small.
+// q must be complete (barrier) on entry; t1 and t2 are work rows; t1 is
+// free again on return, t2 only after a barrier.
 template <int M, int BE, typename Real>
 __device__ __forceinline__ void modal_partials(const StageArgs<M, Real>& a, const Real* q, Real* t1, Real* t2,
-                                               Real& tot, Real& c1, Real& c2, Real& s1, Real& s2) {
+                                               Real& tot, Real& c1, Real& s1, Real& sh2) {
     constexpr int M2 = M * M, M3 = M2 * M;
     const int tid = threadIdx.x;
     const int el = tid / M2, line = tid - el * M2;
@@ -82,36 +82,35 @@ __device__ __forceinline__ void modal_partials(const StageArgs<M, Real>& a, cons
         Real v[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) v[k] = t2[base + swz<M>(ta + M * tb + M2 * k)];
-        tot = c1 = c2 = s1 = s2 = Real(0);
+        tot = c1 = s1 = sh2 = Real(0);
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
 #pragma unroll
             for (int k = 0; k < M; ++k) s += a.op.Vinv[r][k] * v[k];
             const Real e2 = s * s;
+            const int top = (ta > tb ? ta : tb) > r ? (ta > tb ? ta : tb) : r;
             tot += e2;
-            if (ta < M - 1 && tb < M - 1 && r < M - 1)
-                c1 += e2;
-            else
+            if (top == M - 1)
                 s1 += e2;
-            if (ta < M - 2 && tb < M - 2 && r < M - 2)
-                c2 += e2;
             else
-                s2 += e2;
+                c1 += e2;
+            if (top == M - 2) sh2 += e2;
         }
     }
 }
 
-// Blending factor from the Persson-Peraire energy max((tot-c1)/tot,
-// (tot-c2)/tot): logistic ramp with sharpness s = log(9999) around the
-// threshold 0.5*10^(-1.8*M^0.25), values below alpha_min set to 0 (above
-// 1-alpha_min to 1), clipped at alpha_max.
+// Blending factor from the modal energy of Hennemann et al. (JCP 426, 2021,
+// eq. 48): max(top shell / total, second shell / energy below the top
+// shell); logistic ramp with sharpness s = log(9999) around the threshold
+// 0.5*10^(-1.8*M^0.25), values below alpha_min set to 0 (above 1-alpha_min
+// to 1), clipped at alpha_max.
 template <int M, typename Real>
-__device__ __forceinline__ double pp_alpha(const StageArgs<M, Real>& a, Real tot, Real c1, Real c2) {
+__device__ __forceinline__ double pp_alpha(const StageArgs<M, Real>& a, Real tot, Real s1, Real c1, Real sh2) {
     double energy = 0.0;
     if (tot > Real(0)) {
-        const double f1 = double(tot - c1) / double(tot);
-        const double f2 = double(tot - c2) / double(tot);
+        const double f1 = double(s1) / double(tot);
+        const double f2 = c1 > Real(0) ? double(sh2) / double(c1) : 0.0;
         energy = f1 > f2 ? f1 : f2;
     }
     double alpha = 1.0 / (1.0 + exp(-a.ind_s / a.ind_threshold * (energy - a.ind_threshold)));
@@ -148,23 +147,26 @@ __device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long l
         }
     }
     __syncthreads();
-    Real tot, c1, c2, s1, s2;
-    modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, c2, s1, s2);
-    // q (row 2) is dead after the x contraction; rows 2..4 take the partials
-    s_acc[2][tid] = tot;
+    Real tot, c1, s1, sh2;
+    modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, s1, sh2);
+    // rows 0 (t1) and 2 (q) are dead after the y contraction; rows 0, 2..4
+    // take the partials
+    s_acc[0][tid] = tot;
+    s_acc[2][tid] = s1;
     s_acc[3][tid] = c1;
-    s_acc[4][tid] = c2;
+    s_acc[4][tid] = sh2;
     __syncthreads();
     const int el = tid / M2, line = tid - el * M2;
     Real alpha_el = Real(0);
     if (line == 0) {
-        Real st = Real(0), s1 = Real(0), s2 = Real(0);
+        Real st = Real(0), t1 = Real(0), tc = Real(0), t2 = Real(0);
         for (int l = 0; l < M2; ++l) {
-            st += s_acc[2][el * M2 + l];
-            s1 += s_acc[3][el * M2 + l];
-            s2 += s_acc[4][el * M2 + l];
+            st += s_acc[0][el * M2 + l];
+            t1 += s_acc[2][el * M2 + l];
+            tc += s_acc[3][el * M2 + l];
+            t2 += s_acc[4][el * M2 + l];
         }
-        const double alpha = pp_alpha(a, st, s1, s2);
+        const double alpha = pp_alpha(a, st, t1, tc, t2);
         alpha_el = Real(alpha);
         const long long e = e0 + el;
         if (write_out && a.alpha_out && e < a.nelem) a.alpha_out[e] = alpha;
@@ -237,8 +239,9 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
             if (pass == 0 && !(rho > 0.0 && p > 0.0) && e0 + idx / 64 < a.nelem) bad |= 1u << (idx / 64);
         }
         __syncthreads();
-        Real tot, c1, c2, s1, s2;
-        modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, c2, s1, s2);
+        Real tot, c1, s1, sh2;
+        modal_partials<M, BE, Real>(a, q, s_acc[0], s_acc[1], tot, c1, s1, sh2);
+        Real s2 = s1 + sh2;  // top two shells
         tot = sum16(tot);
         s1 = sum16(s1);
         s2 = sum16(s2);

@@ -172,6 +172,57 @@ inline void build_host_ops(int N, HostOps& op) {
     invert(M, V, op.Vinv);
 }
 
+// 2:1 operators of the non-conforming faces and of refinement/coarsening,
+// per direction and half c (0 = lower, 1 = upper):
+//   P[c][i][p] = l_p((x_i + 2c - 1) / 2)   interpolation of the parent's
+//                nodal values to the nodes of half c (the reference's
+//                rfaceMat / refineMat, kernel/mcfv/3d/kernel_share_mod.f90:36-90);
+//   R[c] = (1/2) Mass^-1 P[c]^T Mass       the L2 projection of a polynomial on
+//                half c back to the parent (cfaceMat / coarseMat, :49, which is
+//                0.5 rfaceMat^T for the reference's equal-weight subcells).
+// Mass is the exact mass matrix of the Lagrange basis, Vinv^T Vinv in the
+// orthonormal Legendre basis.  R[0] P[0] + R[1] P[1] = I (coarsening undoes
+// refinement) and sum_p w_p R[c][p][i] = w_i / 2 (conservative).
+inline void mortar_ops(const HostOps& h, double* P /* [2][M][M] */, double* R /* [2][M][M] */) {
+    const int M = h.M;
+    for (int c = 0; c < 2; ++c)
+        for (int i = 0; i < M; ++i) {
+            const double xs = 0.5 * (h.x[i] + (c ? 1.0 : -1.0));
+            for (int p = 0; p < M; ++p) {
+                double l = 1.0;
+                for (int q = 0; q < M; ++q)
+                    if (q != p) l *= (xs - h.x[q]) / (h.x[p] - h.x[q]);
+                P[(c * M + i) * M + p] = l;
+            }
+        }
+    double mass[kMaxM * kMaxM], minv[kMaxM * kMaxM];
+    for (int p = 0; p < M; ++p)
+        for (int q = 0; q < M; ++q) {
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < M; ++j) {
+                a += h.Vinv[j * M + p] * h.Vinv[j * M + q];
+                b += legendre_hat(j, h.x[p]) * legendre_hat(j, h.x[q]);
+            }
+            mass[p * M + q] = a;
+            minv[p * M + q] = b;
+        }
+    for (int c = 0; c < 2; ++c) {
+        double t[kMaxM * kMaxM];  // P[c]^T Mass
+        for (int a = 0; a < M; ++a)
+            for (int i = 0; i < M; ++i) {
+                double s = 0.0;
+                for (int b = 0; b < M; ++b) s += P[(c * M + b) * M + a] * mass[b * M + i];
+                t[a * M + i] = s;
+            }
+        for (int p = 0; p < M; ++p)
+            for (int i = 0; i < M; ++i) {
+                double s = 0.0;
+                for (int a = 0; a < M; ++a) s += minv[p * M + a] * t[a * M + i];
+                R[(c * M + p) * M + i] = 0.5 * s;
+            }
+    }
+}
+
 template <int M, typename Real>
 nemo::OpTable<M, Real> make_table(const HostOps& h) {
     nemo::OpTable<M, Real> t;

@@ -15,6 +15,7 @@
 
 #include "../../include/nemo_gpu.h"
 #include "host_ops.hpp"
+#include "dg_mortar.hpp"
 
 namespace {
 
@@ -40,6 +41,10 @@ struct Solver {
     double* alpha = nullptr;   // [nelem]
     int32_t* nbr = nullptr;
     double* delta = nullptr;
+    long long nmortar = 0;             // non-conforming (2:1) faces
+    int32_t* mortars = nullptr;        // [nmortar][6]
+    double* mortar_flux = nullptr;     // [5*nmortar][5][M*M]
+    int* counter = nullptr;            // remap fallbacks
     double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
     double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;
@@ -78,6 +83,11 @@ void free_mesh_buffers(Solver& s) {
     (void)hipFree(s.alpha_step);
     s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = s.part = s.alpha_step = nullptr;
     s.nbr = nullptr;
+    (void)hipFree(s.mortars);
+    (void)hipFree(s.mortar_flux);
+    s.mortars = nullptr;
+    s.mortar_flux = nullptr;
+    s.nmortar = 0;
 }
 
 // Per-launch options of the stage kernel.
@@ -105,6 +115,7 @@ nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double
     a.rhs_out = rhs_out;
     a.alpha_out = alpha_out;
     a.nbr = s.nbr;
+    a.mortar_flux = s.mortar_flux;
     a.delta = s.delta;
     a.dt_dev = dt_dev;
     a.glm = s.dt_dev + 3;
@@ -138,11 +149,30 @@ void launch_stage_t(Solver& s, const nemo::StageArgs<M, Real>& a) {
                        s.stream, a);
 }
 
+// Fluxes of the non-conforming faces of the stage input (before the stage kernel).
+template <int M, typename Real>
+void launch_mortar(Solver& s) {
+    nemo::MortarArgs<M, Real> a{};
+    a.u_in = s.u[s.cur];
+    a.flux = s.mortar_flux;
+    a.mortars = s.mortars;
+    a.nelem = s.nelem;
+    a.gas = make_gas<Real>(s.cfg.gamma);
+    a.ph64 = make_phys64(s.cfg);
+    mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
+    const dim3 grid(unsigned(s.nmortar)), block(nemo::MortarShape<M>::NT);
+    if (s.cfg.surface == NEMO_SURF_HLLC)
+        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_HLLC, Real>), grid, block, 0, s.stream, a);
+    else
+        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_RUSANOV, Real>), grid, block, 0, s.stream, a);
+}
+
 template <int M, typename Real>
 int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
                     double* cfl_part, const LaunchOpts& o) {
     const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
+    if (s.nmortar > 0) launch_mortar<M, Real>(s);
     if constexpr (M == 4) {
         // learned indicator, and the training-data pass of any Euler scheme
         if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {
@@ -298,6 +328,101 @@ int status_check(Solver* s) {
     return NEMO_OK;
 }
 
+// Mesh checks shared by the uploads: conforming neighbours in range, every
+// non-conforming face pointing at the one mortar slot that belongs to it.
+int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
+               const int32_t* mortars) {
+    if (!nbr || !delta || nelem <= 0 || nelem > INT32_MAX) return fail(NEMO_ERR_ARG, "invalid mesh arguments");
+    if (nmortar < 0 || (nmortar > 0 && !mortars) || nmortar > INT32_MAX / 5)
+        return fail(NEMO_ERR_ARG, "invalid mortar arguments");
+    if (nmortar > 0 && s.cfg.equations != NEMO_EQ_EULER)
+        return fail(NEMO_ERR_ARG, "non-conforming faces are built for the Euler equations");
+    std::vector<int32_t> owner(size_t(5 * nmortar), -1);
+    for (int64_t i = 0; i < 6 * nelem; ++i) {
+        const int32_t n = nbr[i];
+        if (n >= nelem) return fail(NEMO_ERR_ARG, "neighbour index out of range");
+        if (n < 0) {
+            const int64_t slot = -int64_t(n) - 1;
+            if (slot >= 5 * nmortar) return fail(NEMO_ERR_ARG, "mortar slot out of range");
+            if (owner[size_t(slot)] != -1) return fail(NEMO_ERR_ARG, "mortar slot used by two faces");
+            owner[size_t(slot)] = int32_t(i);
+        }
+    }
+    for (int64_t m = 0; m < nmortar; ++m) {
+        const int32_t* t = mortars + 6 * m;
+        if (t[1] < 0 || t[1] > 5) return fail(NEMO_ERR_ARG, "mortar face out of range");
+        for (int k = 0; k < 6; ++k)
+            if (k != 1 && (t[k] < 0 || t[k] >= nelem)) return fail(NEMO_ERR_ARG, "mortar element out of range");
+        if (owner[size_t(5 * m)] != t[0] * 6 + t[1]) return fail(NEMO_ERR_ARG, "mortar big face does not match nbr");
+        for (int k = 0; k < 4; ++k)
+            if (owner[size_t(5 * m + 1 + k)] != t[2 + k] * 6 + (t[1] ^ 1))
+                return fail(NEMO_ERR_ARG, "mortar small face does not match nbr");
+    }
+    for (int64_t i = 0; i < 3 * nelem; ++i)
+        if (!(delta[i] > 0.0)) return fail(NEMO_ERR_ARG, "element size must be positive");
+    return NEMO_OK;
+}
+
+// Device copies of a checked mesh; state-sized buffers are reallocated when
+// the element count changes (the state itself then needs an upload or a remap).
+int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
+             const int32_t* mortars) {
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (nelem != s.nelem) {
+        free_mesh_buffers(s);
+        s.nelem = nelem;
+        const size_t bytes = s.state_doubles() * sizeof(double);
+        HIP_TRY(hipMalloc(&s.u[0], bytes));
+        HIP_TRY(hipMalloc(&s.u[1], bytes));
+        HIP_TRY(hipMalloc(&s.r0, bytes));
+        HIP_TRY(hipMalloc(&s.r1, bytes));
+        HIP_TRY(hipMalloc(&s.rhs, bytes));
+        HIP_TRY(hipMalloc(&s.alpha, size_t(nelem) * sizeof(double)));
+        HIP_TRY(hipMalloc(&s.nbr, size_t(nelem) * 6 * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&s.delta, size_t(nelem) * 3 * sizeof(double)));
+        HIP_TRY(hipMalloc(&s.part, size_t(nblocks_for(s)) * 4 * sizeof(double)));
+        HIP_TRY(hipMalloc(&s.alpha_step, size_t(nelem) * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(s.alpha_step, 0, size_t(nelem) * sizeof(double), s.stream));
+        HIP_TRY(hipMemsetAsync(s.r0, 0, bytes, s.stream));
+        HIP_TRY(hipMemsetAsync(s.r1, 0, bytes, s.stream));
+    }
+    if (nmortar != s.nmortar) {
+        (void)hipFree(s.mortars);
+        (void)hipFree(s.mortar_flux);
+        s.mortars = nullptr;
+        s.mortar_flux = nullptr;
+        s.nmortar = nmortar;
+        if (nmortar > 0) {
+            HIP_TRY(hipMalloc(&s.mortars, size_t(nmortar) * 6 * sizeof(int32_t)));
+            HIP_TRY(hipMalloc(&s.mortar_flux, size_t(nmortar) * 25 * size_t(s.M * s.M) * sizeof(double)));
+        }
+    }
+    if (!s.counter) HIP_TRY(hipMalloc(&s.counter, sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(s.nbr, nbr, size_t(nelem) * 6 * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+    HIP_TRY(hipMemcpyAsync(s.delta, delta, size_t(nelem) * 3 * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    if (nmortar > 0)
+        HIP_TRY(hipMemcpyAsync(s.mortars, mortars, size_t(nmortar) * 6 * sizeof(int32_t), hipMemcpyHostToDevice,
+                               s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.have_mesh = true;
+    return NEMO_OK;
+}
+
+template <int M>
+void launch_remap(Solver& s, double* u_new, const int32_t* d_units, int64_t nunits, int64_t nelem_new) {
+    nemo::RemapArgs<M> a{};
+    a.u_old = s.u[s.cur];
+    a.u_new = u_new;
+    a.units = d_units;
+    a.nold = s.nelem;
+    a.nnew = nelem_new;
+    a.ph64 = make_phys64(s.cfg);
+    mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
+    for (int i = 0; i < M; ++i) a.w[i] = s.ops.w[i];
+    a.fallbacks = s.counter;
+    hipLaunchKernelGGL((nemo::remap_kernel<M>), dim3(unsigned(nunits)), dim3(nemo::RemapShape<M>::NT), 0, s.stream, a);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -390,6 +515,7 @@ int nemo_gpu_destroy(void* handle) {
     free_mesh_buffers(*s);
     (void)hipFree(s->dt_dev);
     (void)hipFree(s->status);
+    (void)hipFree(s->counter);
     (void)hipFree(s->mlp_p);
     (void)hipFree(s->mlp_inf);
     (void)hipFree(s->mlp_m);
@@ -406,36 +532,75 @@ int nemo_gpu_destroy(void* handle) {
 }
 
 int nemo_gpu_upload_mesh(void* handle, int64_t nelem, const int32_t* nbr, const double* delta) {
+    return nemo_gpu_upload_mesh_amr(handle, nelem, nbr, delta, 0, nullptr);
+}
+
+int nemo_gpu_upload_mesh_amr(void* handle, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
+                             const int32_t* mortars) {
     Solver* s = static_cast<Solver*>(handle);
-    if (!s || !nbr || !delta || nelem <= 0) return fail(NEMO_ERR_ARG, "invalid mesh arguments");
-    for (int64_t i = 0; i < 6 * nelem; ++i)
-        if (nbr[i] < 0 || nbr[i] >= nelem) return fail(NEMO_ERR_ARG, "neighbour index out of range");
-    for (int64_t i = 0; i < 3 * nelem; ++i)
-        if (!(delta[i] > 0.0)) return fail(NEMO_ERR_ARG, "element size must be positive");
+    if (!s) return fail(NEMO_ERR_ARG, "null handle");
+    if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
-    if (nelem != s->nelem) {
-        free_mesh_buffers(*s);
-        s->nelem = nelem;
-        const size_t bytes = s->state_doubles() * sizeof(double);
-        HIP_TRY(hipMalloc(&s->u[0], bytes));
-        HIP_TRY(hipMalloc(&s->u[1], bytes));
-        HIP_TRY(hipMalloc(&s->r0, bytes));
-        HIP_TRY(hipMalloc(&s->r1, bytes));
-        HIP_TRY(hipMalloc(&s->rhs, bytes));
-        HIP_TRY(hipMalloc(&s->alpha, size_t(nelem) * sizeof(double)));
-        HIP_TRY(hipMalloc(&s->nbr, size_t(nelem) * 6 * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&s->delta, size_t(nelem) * 3 * sizeof(double)));
-        HIP_TRY(hipMalloc(&s->part, size_t(nblocks_for(*s)) * 4 * sizeof(double)));
-        HIP_TRY(hipMalloc(&s->alpha_step, size_t(nelem) * sizeof(double)));
-        HIP_TRY(hipMemsetAsync(s->alpha_step, 0, size_t(nelem) * sizeof(double), s->stream));
-        HIP_TRY(hipMemsetAsync(s->r0, 0, bytes, s->stream));
-        HIP_TRY(hipMemsetAsync(s->r1, 0, bytes, s->stream));
-        s->have_state = false;
+    if (nelem != s->nelem) s->have_state = false;
+    return set_mesh(*s, nelem, nbr, delta, nmortar, mortars);
+}
+
+int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
+                    const int32_t* mortars, int64_t nunits, const int32_t* units, int64_t* fallbacks) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (s->cfg.equations != NEMO_EQ_EULER) return fail(NEMO_ERR_ARG, "mesh adaptation is built for the Euler equations");
+    if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
+    if (nunits <= 0 || !units) return fail(NEMO_ERR_ARG, "invalid replacement units");
+    // the units must pair every old element with every new one, in order
+    long long io = 0, jn = 0;
+    for (int64_t u = 0; u < nunits; ++u) {
+        const int32_t* t = units + 3 * u;
+        if (t[1] != io || t[2] != jn) return fail(NEMO_ERR_ARG, "replacement units out of order");
+        switch (t[0]) {
+        case nemo::UNIT_COPY: io += 1; jn += 1; break;
+        case nemo::UNIT_REFINE: io += 1; jn += 8; break;
+        case nemo::UNIT_COARSEN: io += 8; jn += 1; break;
+        default: return fail(NEMO_ERR_ARG, "unknown replacement unit kind");
+        }
+        if (io > s->nelem || jn > nelem) return fail(NEMO_ERR_ARG, "replacement units exceed the meshes");
     }
-    HIP_TRY(hipMemcpyAsync(s->nbr, nbr, size_t(nelem) * 6 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->delta, delta, size_t(nelem) * 3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    if (io != s->nelem || jn != nelem) return fail(NEMO_ERR_ARG, "replacement units do not cover both meshes");
+
+    const size_t new_bytes = size_t(s->V) * size_t(nelem) * size_t(s->M3) * sizeof(double);
+    double* u_new = nullptr;
+    int32_t* d_units = nullptr;
+    HIP_TRY(hipMalloc(&u_new, new_bytes));
+    HIP_TRY(hipMalloc(&d_units, size_t(nunits) * 3 * sizeof(int32_t)));
+    HIP_TRY(hipMemcpyAsync(d_units, units, size_t(nunits) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->counter, 0, sizeof(int), s->stream));
+    int rc = NEMO_OK;
+    switch (s->M) {
+    case 3: launch_remap<3>(*s, u_new, d_units, nunits, nelem); break;
+    case 4: launch_remap<4>(*s, u_new, d_units, nunits, nelem); break;
+    case 5: launch_remap<5>(*s, u_new, d_units, nunits, nelem); break;
+    default: rc = fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    }
+    if (rc == NEMO_OK) {
+        HIP_TRY(hipGetLastError());
+        int nfb = 0;
+        HIP_TRY(hipMemcpyAsync(&nfb, s->counter, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        if (fallbacks) *fallbacks = nfb;
+        rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars);
+    }
+    (void)hipFree(d_units);
+    if (rc != NEMO_OK) {
+        (void)hipFree(u_new);
+        return rc;
+    }
+    (void)hipFree(s->u[s->cur]);
+    s->u[s->cur] = u_new;
+    HIP_TRY(hipMemsetAsync(s->r0, 0, new_bytes, s->stream));
+    HIP_TRY(hipMemsetAsync(s->r1, 0, new_bytes, s->stream));
+    HIP_TRY(hipMemsetAsync(s->alpha_step, 0, size_t(nelem) * sizeof(double), s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    s->have_mesh = true;
+    s->have_state = true;
     return NEMO_OK;
 }
 

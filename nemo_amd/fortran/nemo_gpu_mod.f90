@@ -76,6 +76,31 @@ interface
         integer(c_int)                    :: rc
     end function
 
+    !! adaptive mesh: sides with nquads = 4 as mortars(6, nmortar) =
+    !! (big quad, its face, small quads 0..3), nbr = -(5m+1) / -(5m+2+k) on them
+    function nemo_gpu_upload_mesh_amr(handle, nelem, nbr, delta, nmortar, mortars) result(rc) &
+            bind(c, name='nemo_gpu_upload_mesh_amr')
+        import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+        type(c_ptr), value                :: handle
+        integer(c_int64_t), value         :: nelem, nmortar
+        integer(c_int32_t), intent(in)    :: nbr(*), mortars(*)
+        real(c_double), intent(in)        :: delta(*)
+        integer(c_int)                    :: rc
+    end function
+
+    !! state transfer of mesh_adapt_wrapped: units(3, nunits) = (kind, first
+    !! old quad, first new quad), kind 0 copy, 1 refine, 2 coarsen (0-based)
+    function nemo_gpu_remesh(handle, nelem, nbr, delta, nmortar, mortars, nunits, units, fallbacks) result(rc) &
+            bind(c, name='nemo_gpu_remesh')
+        import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+        type(c_ptr), value                :: handle
+        integer(c_int64_t), value         :: nelem, nmortar, nunits
+        integer(c_int32_t), intent(in)    :: nbr(*), mortars(*), units(*)
+        real(c_double), intent(in)        :: delta(*)
+        integer(c_int64_t), intent(out)   :: fallbacks
+        integer(c_int)                    :: rc
+    end function
+
     function nemo_gpu_upload_state(handle, u) result(rc) bind(c, name='nemo_gpu_upload_state')
         import :: c_int, c_ptr, c_double
         type(c_ptr), value         :: handle

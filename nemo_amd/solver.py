@@ -105,10 +105,28 @@ class Solver:
 
     # -- mesh and state (mesh_alloc / payload upload) -----------------------
     def upload_mesh(self, mesh: Mesh) -> None:
+        """Connectivity of a uniform mesh or of an adaptive forest (nemo_amd.amr.Forest)."""
         nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32)
         delta = np.ascontiguousarray(mesh.delta, dtype=np.float64)
-        self._call("nemo_gpu_upload_mesh", self._h, ctypes.c_int64(mesh.nelem), iptr(nbr), dptr(delta))
+        mortars = np.ascontiguousarray(getattr(mesh, "mortars", np.zeros((0, 6))), dtype=np.int32)
+        self._call("nemo_gpu_upload_mesh_amr", self._h, ctypes.c_int64(mesh.nelem), iptr(nbr), dptr(delta),
+                   ctypes.c_int64(mortars.shape[0]), iptr(mortars))
         self.nelem = mesh.nelem
+
+    def remesh(self, forest, units: np.ndarray) -> int:
+        """Move the device state onto a new forest (kernel_amr_refine/coarsen on
+        the GPU); units from nemo_amd.amr.replacement_units.  Returns the
+        number of units that fell back to the element mean."""
+        nbr = np.ascontiguousarray(forest.nbr, dtype=np.int32)
+        delta = np.ascontiguousarray(forest.delta, dtype=np.float64)
+        mortars = np.ascontiguousarray(forest.mortars, dtype=np.int32)
+        units = np.ascontiguousarray(units, dtype=np.int32)
+        fb = ctypes.c_int64(0)
+        self._call("nemo_gpu_remesh", self._h, ctypes.c_int64(forest.nelem), iptr(nbr), dptr(delta),
+                   ctypes.c_int64(mortars.shape[0]), iptr(mortars), ctypes.c_int64(units.shape[0]), iptr(units),
+                   ctypes.byref(fb))
+        self.nelem = forest.nelem
+        return int(fb.value)
 
     def set_state(self, u: np.ndarray) -> None:
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -924,21 +924,31 @@ static double indicator_element(const orc_cfg *c, const orc_ops *op, const doubl
                 for (int j = 0; j < M; ++j) s += op->Vinv[b * M + j] * t1[a + M * j + M * M * k];
                 t2[a + M * b + M * M * k] = s;
             }
-    double total = 0.0, clip1 = 0.0, clip2 = 0.0;
+    /* Hennemann et al. (JCP 426, 2021, eq. 48): energy = max(E_top / E,
+     * E_second / E_clip1) with E_clip1 the energy of the modes whose indices
+     * are all below M-1, E_top = E - E_clip1 (largest index M-1) and
+     * E_second = E_clip1 - E_clip2 (largest index M-2); the shells are summed
+     * directly, not as differences of the clipped sums. */
+    double total = 0.0, clip1 = 0.0, top = 0.0, second = 0.0;
     for (int cc = 0; cc < M; ++cc)
         for (int b = 0; b < M; ++b)
             for (int a = 0; a < M; ++a) {
                 double s = 0.0;
                 for (int k = 0; k < M; ++k) s += op->Vinv[cc * M + k] * t2[a + M * b + M * M * k];
                 double e2 = s * s;
+                int mx = a > b ? a : b;
+                mx = mx > cc ? mx : cc;
                 total += e2;
-                if (a < M - 1 && b < M - 1 && cc < M - 1) clip1 += e2;
-                if (a < M - 2 && b < M - 2 && cc < M - 2) clip2 += e2;
+                if (mx == M - 1)
+                    top += e2;
+                else
+                    clip1 += e2;
+                if (mx == M - 2) second += e2;
             }
     double energy = 0.0;
     if (total > 0.0) {
-        double f1 = (total - clip1) / total;
-        double f2 = (total - clip2) / total;
+        double f1 = top / total;
+        double f2 = clip1 > 0.0 ? second / clip1 : 0.0;
         energy = f1 > f2 ? f1 : f2;
     }
     if (energy_out) *energy_out = energy;
@@ -978,7 +988,7 @@ void orc_indicator(const orc_cfg *c, int64_t nelem, const double *u, double *alp
  *              rhs_i = -(2/dx) (F_{i+1/2} - F_{i-1/2}) / w_i
  *  blended:    rhs = (1-alpha) rhs_DG + alpha rhs_FV. */
 static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const int32_t *nbr,
-                        const double *delta, const double *u, double alpha, double *out) {
+                        const double *delta, const double *u, double alpha, const double *mflux, double *out) {
     const int M = op->M, M3 = op->M3, V = orc_nvars(c);
     const int mhd = c->eq == ORC_EQ_MHD;
     const double *ue = u + e * V * M3;
@@ -997,12 +1007,20 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                 double ul[MAXM][MAXV], unm[MAXV], unp[MAXV], fm[MAXV], fp[MAXV];
                 for (int l = 0; l < M; ++l)
                     for (int h = 0; h < V; ++h) ul[l][h] = ue[h * M3 + node_index(M, d, l, a, b)];
-                for (int h = 0; h < V; ++h) {
-                    unm[h] = u[em * V * M3 + h * M3 + node_index(M, d, M - 1, a, b)];
-                    unp[h] = u[ep * V * M3 + h * M3 + node_index(M, d, 0, a, b)];
+                /* a face on a non-conforming interface takes its flux
+                 * from the mortar pass (nbr = -(slot+1)) */
+                if (em >= 0) {
+                    for (int h = 0; h < V; ++h) unm[h] = u[em * V * M3 + h * M3 + node_index(M, d, M - 1, a, b)];
+                    phys_riemann(c, c->surface, d, unm, ul[0], fm);
+                } else {
+                    for (int h = 0; h < V; ++h) fm[h] = mflux[((-em - 1) * V + h) * op->M2 + a + M * b];
                 }
-                phys_riemann(c, c->surface, d, unm, ul[0], fm);
-                phys_riemann(c, c->surface, d, ul[M - 1], unp, fp);
+                if (ep >= 0) {
+                    for (int h = 0; h < V; ++h) unp[h] = u[ep * V * M3 + h * M3 + node_index(M, d, 0, a, b)];
+                    phys_riemann(c, c->surface, d, ul[M - 1], unp, fp);
+                } else {
+                    for (int h = 0; h < V; ++h) fp[h] = mflux[((-ep - 1) * V + h) * op->M2 + a + M * b];
+                }
                 if (mhd) {
                     /* Powell/GLM terms at the faces with the element's own
                      * node as inner state and the face average (the
@@ -1103,17 +1121,133 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
         for (int n = 0; n < M3; ++n) out[GLMP * M3 + n] = out[GLMP * M3 + n] - c->glm_alpha * ue[GLMP * M3 + n];
 }
 
+/* 2:1 interpolation and projection, per direction and half c:
+ * P_c interpolates the parent's nodal polynomial to the LGL nodes of half c
+ * (the reference's rfaceMat/refineMat, kernel/mcfv/3d/kernel_share_mod.f90:
+ * 36-90, with Lagrange interpolation of utils/lagrange_mod.f90:52-71 in
+ * place of the subcell injection of the FV kernel); R_c = (1/2) Mass^-1
+ * P_c^T Mass is the L2 projection back (cfaceMat = 0.5 rfaceMat^T at :49 is
+ * the equal-weight case), Mass = Vinv^T Vinv the exact mass matrix of the
+ * Lagrange basis and Mass^-1 = V V^T (orthonormal Legendre Vandermonde). */
+void orc_mortar_ops(int N, double *P, double *R) {
+    const int M = N + 1;
+    double x[MAXM], w[MAXM], V[MAXM * MAXM], Vi[MAXM * MAXM], mass[MAXM * MAXM], minv[MAXM * MAXM];
+    orc_lgl_nodes_weights(N, x, w);
+    orc_vandermonde(N, x, V, Vi);
+    for (int c = 0; c < 2; ++c)
+        for (int i = 0; i < M; ++i) {
+            const double xs = 0.5 * x[i] + (c ? 0.5 : -0.5);
+            for (int p = 0; p < M; ++p) {
+                double l = 1.0;
+                for (int q = 0; q < M; ++q)
+                    if (q != p) l = l * (xs - x[q]) / (x[p] - x[q]);
+                P[(c * M + i) * M + p] = l;
+            }
+        }
+    for (int p = 0; p < M; ++p)
+        for (int q = 0; q < M; ++q) {
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < M; ++j) {
+                a += Vi[j * M + p] * Vi[j * M + q];
+                b += V[p * M + j] * V[q * M + j];
+            }
+            mass[p * M + q] = a;
+            minv[p * M + q] = b;
+        }
+    for (int c = 0; c < 2; ++c)
+        for (int p = 0; p < M; ++p)
+            for (int i = 0; i < M; ++i) {
+                double s = 0.0;
+                for (int a = 0; a < M; ++a) {
+                    double t = 0.0;
+                    for (int b = 0; b < M; ++b) t += P[(c * M + b) * M + a] * mass[b * M + i];
+                    s += minv[p * M + a] * t;
+                }
+                R[(c * M + p) * M + i] = 0.5 * s;
+            }
+}
+
+/* Mortar sequence of kernel_fill_faceflux_cb (kernel/dgsem/weakform/2d/
+ * kernel_flux_mod.f90:58-147) on the 3-D face arrays of kernel/mcfv/3d/
+ * kernel_flux_mod.f90:100-121: big face values -> the four half faces
+ * (P along both tangential axes), Riemann problem per small-face node with
+ * the x+/y+/z+ side as u_L, small faces keep their fluxes, the big face
+ * takes sum_k R (x) R of them. */
+void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux) {
+    (void)nelem;
+    const int M = c->N + 1, M2 = M * M, M3 = M2 * M, V = 5;
+    double P[2 * MAXM * MAXM], R[2 * MAXM * MAXM];
+    orc_mortar_ops(c->N, P, R);
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < c->nmortar; ++m) {
+        const int32_t *t = c->mortars + 6 * m;
+        const int64_t big = t[0];
+        const int fb = t[1], d = fb / 2, big_left = fb % 2;
+        double ubig[MAXV][MAXM * MAXM], fk[4][MAXV][MAXM * MAXM];
+        for (int h = 0; h < V; ++h)
+            for (int q = 0; q < M; ++q)
+                for (int p = 0; p < M; ++p)
+                    ubig[h][p + M * q] = u[big * V * M3 + h * M3 + node_index(M, d, big_left ? M - 1 : 0, p, q)];
+        for (int k = 0; k < 4; ++k) {
+            const int ka = k % 2, kb = k / 2;
+            const int64_t sm = t[2 + k];
+            for (int tb = 0; tb < M; ++tb)
+                for (int ta = 0; ta < M; ++ta) {
+                    double us[MAXV], ub[MAXV], f[MAXV];
+                    for (int h = 0; h < V; ++h) {
+                        us[h] = u[sm * V * M3 + h * M3 + node_index(M, d, big_left ? 0 : M - 1, ta, tb)];
+                        double acc = 0.0;
+                        for (int q = 0; q < M; ++q) {
+                            double r = 0.0;
+                            for (int p = 0; p < M; ++p) r += P[(ka * M + ta) * M + p] * ubig[h][p + M * q];
+                            acc += P[(kb * M + tb) * M + q] * r;
+                        }
+                        ub[h] = acc;
+                    }
+                    if (big_left)
+                        phys_riemann(c, c->surface, d, ub, us, f);
+                    else
+                        phys_riemann(c, c->surface, d, us, ub, f);
+                    for (int h = 0; h < V; ++h) {
+                        fk[k][h][ta + M * tb] = f[h];
+                        flux[((5 * m + 1 + k) * V + h) * M2 + ta + M * tb] = f[h];
+                    }
+                }
+        }
+        for (int h = 0; h < V; ++h)
+            for (int q = 0; q < M; ++q)
+                for (int p = 0; p < M; ++p) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int ka = k % 2, kb = k / 2;
+                        for (int tb = 0; tb < M; ++tb) {
+                            double r = 0.0;
+                            for (int ta = 0; ta < M; ++ta) r += R[(ka * M + p) * M + ta] * fk[k][h][ta + M * tb];
+                            acc += R[(kb * M + q) * M + tb] * r;
+                        }
+                    }
+                    flux[((5 * m) * V + h) * M2 + p + M * q] = acc;
+                }
+    }
+}
+
 void orc_rhs(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, const double *u,
              const double *alpha, double *rhs) {
     orc_ops op;
     build_ops(c->N, &op);
     const int V = orc_nvars(c);
+    double *mflux = NULL;
+    if (c->nmortar > 0) {
+        mflux = (double *)malloc(sizeof(double) * (size_t)(5 * c->nmortar) * 5 * (size_t)op.M2);
+        orc_mortar_fluxes(c, nelem, u, mflux);
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nelem; ++e) {
         double a = 0.0;
         if (c->indicator != ORC_IND_NONE) a = alpha ? alpha[e] : indicator_element(c, &op, u + e * V * op.M3, NULL);
-        rhs_element(c, &op, e, nbr, delta, u, a, rhs + e * V * op.M3);
+        rhs_element(c, &op, e, nbr, delta, u, a, mflux, rhs + e * V * op.M3);
     }
+    free(mflux);
 }
 
 /* CFL rule: kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-121 with the

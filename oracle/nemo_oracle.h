@@ -48,6 +48,12 @@ typedef struct {
     double mu0;         /* MHD only */
     double glm_ch;      /* MHD only: GLM cleaning speed */
     double glm_alpha;   /* MHD only: GLM damping */
+    int64_t nmortar;    /* Euler only: non-conforming (2:1) interfaces */
+    const int32_t *mortars; /* [nmortar][6]: big element, its face, small_0..small_3;
+                               faces on mortar m carry nbr = -(5m+1) (big) and
+                               -(5m+2+k) (small_k); small_k covers half k&1 of the
+                               big face along its first tangential axis and half
+                               k>>1 along the second (axes in increasing order) */
 } orc_cfg;
 
 /* ---- operators (utils/interpol_mod.f90, utils/linalg_mod.f90) ---------- */
@@ -85,6 +91,12 @@ void orc_phys_nc(const orc_cfg *c, int dir, const double *uI, const double *uL,
                  const double *uR, double *f);
 double orc_phys_lambda_max(const orc_cfg *c, int dir, const double *u);
 int orc_phys_isvalid(const orc_cfg *c, const double *u);
+
+/* ---- 2:1 operators (rfaceMat/cfaceMat, refineMat/coarseMat) ------------- */
+/* P[(c*M+i)*M+p] = l_p((x_i + 2c - 1)/2), R[(c*M+p)*M+i] = w_i P[c][i][p] / (2 w_p) */
+void orc_mortar_ops(int N, double *P, double *R);
+/* Face fluxes of every mortar slot: flux[slot][var][M*M] (Euler). */
+void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux);
 
 /* ---- element operators -------------------------------------------------- */
 int orc_nvars(const orc_cfg *c);

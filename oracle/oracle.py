@@ -46,6 +46,8 @@ class Cfg(ctypes.Structure):
         ("mu0", ctypes.c_double),
         ("glm_ch", ctypes.c_double),
         ("glm_alpha", ctypes.c_double),
+        ("nmortar", ctypes.c_int64),
+        ("mortars", ctypes.c_void_p),
     ]
 
 
@@ -54,7 +56,19 @@ def make_cfg(N=3, eq=EQ_EULER, volume=VOL_WEAK, surface=SURF_RUSANOV,
              alpha_max=0.5, alpha_min=0.001, cfl=0.5, mu0=1.0, glm_ch=0.0,
              glm_alpha=0.0) -> Cfg:
     return Cfg(N, eq, volume, surface, indicator, timedisc, repair, 0, gamma,
-               alpha_max, alpha_min, cfl, mu0, glm_ch, glm_alpha)
+               alpha_max, alpha_min, cfl, mu0, glm_ch, glm_alpha, 0, None)
+
+
+def with_mortars(cfg: Cfg, mortars) -> Cfg:
+    """Copy of cfg that carries a mortar table (kept alive with the copy)."""
+    if cfg.eq != EQ_EULER and mortars is not None and len(mortars):
+        raise ValueError("non-conforming faces are restated for the Euler equations only")
+    c = Cfg.from_buffer_copy(cfg)
+    arr = np.ascontiguousarray(mortars if mortars is not None else np.zeros((0, 6)), dtype=np.int32)
+    c._mortars = arr
+    c.nmortar = arr.shape[0]
+    c.mortars = arr.ctypes.data if arr.shape[0] else None
+    return c
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -92,6 +106,8 @@ def lib():
         L.orc_stage.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, _P, _P, ctypes.c_int, ctypes.c_double]
         L.orc_run.argtypes = [cp, ctypes.c_int64, _PI, _P, _P, ctypes.c_int, _P]
         L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_double, _P, _P]
+        L.orc_mortar_ops.argtypes = [ctypes.c_int, _P, _P]
+        L.orc_mortar_fluxes.argtypes = [cp, ctypes.c_int64, _P, _P]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_mhd_pressure.restype = ctypes.c_double
         L.orc_mhd_fastspeed.restype = ctypes.c_double
@@ -258,6 +274,72 @@ def rhs(cfg: Cfg, nbr: np.ndarray, delta: np.ndarray, u: np.ndarray,
     lib().orc_rhs(ctypes.byref(cfg), u.shape[0], _iptr(np.ascontiguousarray(nbr, dtype=np.int32)),
                   _ptr(np.ascontiguousarray(delta, dtype=np.float64)), _ptr(u), ap, _ptr(out))
     return out
+
+
+def mortar_ops(N: int):
+    """(P, R), each [2, M, M]: 2:1 interpolation and its projection."""
+    M = N + 1
+    P = np.zeros((2, M, M))
+    R = np.zeros((2, M, M))
+    lib().orc_mortar_ops(ctypes.c_int(N), _ptr(P), _ptr(R))
+    return P, R
+
+
+def mortar_fluxes(cfg: Cfg, u: np.ndarray) -> np.ndarray:
+    """Face fluxes of every mortar slot, [5*nmortar, 5, M*M]."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    M = cfg.N + 1
+    out = np.zeros((5 * cfg.nmortar, 5, M * M))
+    lib().orc_mortar_fluxes(ctypes.byref(cfg), u.shape[0], _ptr(u), _ptr(out))
+    return out
+
+
+UNIT_COPY, UNIT_REFINE, UNIT_COARSEN = 0, 1, 2
+
+
+def _valid_euler(gamma: float, v: np.ndarray) -> bool:
+    """rho > 0 and p > 0 at every node of v [..., 5, M^3] (equations_mod.f90:337-344)."""
+    rho = v[..., 0, :]
+    p = (gamma - 1.0) * (v[..., 4, :] - 0.5 * (v[..., 1, :] ** 2 + v[..., 2, :] ** 2 + v[..., 3, :] ** 2) / rho)
+    return bool(np.all(rho > 0.0) and np.all(p > 0.0))
+
+
+def remap(N: int, gamma: float, u_old: np.ndarray, units: np.ndarray, nnew: int):
+    """State transfer of one adaptation: kernel_amr_refine / kernel_amr_coarsen
+    (kernel/mcfv/3d/kernel_amr_mod.f90:12-70) as tensor products of the 2:1
+    operators per direction, child c = x + 2y + 4z; a unit whose result has a
+    nonpositive density or pressure takes the element mean instead (the
+    positivity fallback of setup_amr_replace_refine/coarsen,
+    setups/euler/3d/sedov/setup_amr_mod.f90:76-130, reduced to its last,
+    first-order step).  Returns (u_new [nnew, 5, M^3], number of fallbacks)."""
+    M = N + 1
+    P, R = mortar_ops(N)
+    _, w = lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1) / 8.0  # (k, j, i) order
+    nold = u_old.shape[0]
+    uo = u_old.reshape(nold, 5, M, M, M)       # node axes (k, j, i)
+    out = np.zeros((nnew, 5, M, M, M))
+    fallbacks = 0
+    for kind, io, jn in units:
+        if kind == UNIT_COPY:
+            out[jn] = uo[io]
+        elif kind == UNIT_REFINE:
+            ch = np.stack([np.einsum("ka,jb,ic,vabc->vkji", P[c >> 2], P[(c >> 1) & 1], P[c & 1], uo[io])
+                           for c in range(8)])
+            if not _valid_euler(gamma, ch.reshape(8, 5, -1)):
+                mean = uo[io].reshape(5, -1) @ w3
+                ch[:] = mean[None, :, None, None, None]
+                fallbacks += 1
+            out[jn:jn + 8] = ch
+        else:
+            par = sum(np.einsum("ka,jb,ic,vabc->vkji", R[c >> 2], R[(c >> 1) & 1], R[c & 1], uo[io + c])
+                      for c in range(8))
+            if not _valid_euler(gamma, par.reshape(5, -1)):
+                mean = par.reshape(5, -1) @ w3
+                par = np.broadcast_to(mean[:, None, None, None], par.shape).copy()
+                fallbacks += 1
+            out[jn] = par
+    return out.reshape(nnew, 5, M ** 3), fallbacks
 
 
 def calc_timestep(cfg: Cfg, delta: np.ndarray, u: np.ndarray):

@@ -1,0 +1,271 @@
+"""Adaptive mesh (nemo_amd.amr) and the oracle's non-conforming faces and
+state transfer, on the CPU.
+
+The forest is checked against geometry (face planes, half-face positions,
+2:1 balance, symmetric conforming neighbours) and against the uniform mesh
+module; the oracle's 2:1 operators against Gauss-quadrature L2 projections;
+its mortar fluxes against closed forms: free-stream preservation,
+conservation, and an exactly representable state whose RHS is known
+(-grad p at rest), which fails for a wrong half, orientation, slot or
+transposed projection."""
+import numpy as np
+import pytest
+
+from nemo_amd import amr, indicator_mlp, mesh as meshmod
+from oracle import oracle as O
+
+
+def _forest(seed=0, lmin=2, lmax=4):
+    """Two rounds of refinement around a point, balanced."""
+    f = amr.Forest.uniform(lmin, maxlevel=lmax)
+    c0 = np.array([0.45, 0.4, 0.5])
+    for r in (0.3, 0.15):
+        cen = f.lower + 0.5 * f.delta
+        f = f.refine(np.linalg.norm(cen - c0, axis=1) < r).balance()
+    return f
+
+
+def test_uniform_forest_matches_uniform_mesh():
+    for level in (1, 2, 3):
+        f = amr.Forest.uniform(level)
+        m = meshmod.uniform_periodic(1 << level)
+        nbr, mort = f.connectivity()
+        assert mort.shape == (0, 6)
+        assert np.array_equal(nbr, m.nbr)
+        assert np.allclose(f.delta, m.delta) and np.allclose(f.lower, m.lower)
+
+
+def test_forest_geometry_and_balance():
+    f = _forest()
+    nbr, mort = f.connectivity()
+    lo, d = f.lower, f.delta
+    assert len(mort) > 0 and len(set(f.level.tolist())) == 3
+    # conforming neighbours: symmetric, same size, touching (periodic)
+    for e in range(f.nelem):
+        for face in range(6):
+            n = nbr[e, face]
+            if n < 0:
+                continue
+            ax, side = face // 2, face % 2
+            assert nbr[n, face ^ 1] == e
+            assert f.level[n] == f.level[e]
+            gap = (lo[n, ax] - (lo[e, ax] + d[e, ax])) if side else (lo[e, ax] - (lo[n, ax] + d[n, ax]))
+            assert abs(gap - round(gap)) < 1e-12
+    # mortars: the four halves tile the big face in the documented order
+    seen = set()
+    for m, (big, face, *sm) in enumerate(mort):
+        ax, side = face // 2, face % 2
+        t1, t2 = amr.TANGENTIAL[ax]
+        assert nbr[big, face] == -(5 * m + 1)
+        plane = lo[big, ax] + (d[big, ax] if side else 0.0)
+        for k, s in enumerate(sm):
+            assert nbr[s, face ^ 1] == -(5 * m + 2 + k)
+            assert f.level[s] == f.level[big] + 1
+            sp = lo[s, ax] + (0.0 if side else d[s, ax])
+            assert abs((sp - plane) - round(sp - plane)) < 1e-12
+            assert abs(lo[s, t1] - (lo[big, t1] + (k & 1) * 0.5 * d[big, t1])) < 1e-12
+            assert abs(lo[s, t2] - (lo[big, t2] + (k >> 1) * 0.5 * d[big, t2])) < 1e-12
+            seen.add((int(s), int(face ^ 1)))
+        seen.add((int(big), int(face)))
+    assert len(seen) == 5 * len(mort) == int((nbr < 0).sum())
+
+
+def test_balance_enforced():
+    f = amr.Forest.uniform(2, maxlevel=5)
+    # refine one leaf three times without balancing -> 2:1 violated
+    for _ in range(3):
+        flags = np.zeros(f.nelem, dtype=bool)
+        flags[np.argmax(f.level)] = True
+        f = f.refine(flags)
+    with pytest.raises(ValueError):
+        f.connectivity()
+    b = f.balance()
+    nbr, _ = b.connectivity()
+    assert b.nelem > f.nelem
+
+
+def test_adapt_units_and_round_trip():
+    f = _forest(lmax=5)
+    cen = f.lower + 0.5 * f.delta
+    ref = np.linalg.norm(cen - 0.45, axis=1) < 0.1
+    g, units = f.adapt(ref, np.zeros(f.nelem, bool))
+    assert units is not None
+    kinds = np.bincount(units[:, 0], minlength=3)
+    assert kinds[amr.UNIT_REFINE] >= ref.sum() and kinds[amr.UNIT_COARSEN] == 0
+    # every old leaf is consumed once, every new leaf produced once
+    no = np.where(units[:, 0] == amr.UNIT_COARSEN, 8, 1).sum()
+    nn = np.where(units[:, 0] == amr.UNIT_REFINE, 8, 1).sum()
+    assert no == f.nelem and nn == g.nelem
+    # coarsening everywhere merges families but never goes below minlevel
+    assert g.family_starts().sum() > 0
+    back, units3 = g.adapt(np.zeros(g.nelem, bool), np.ones(g.nelem, bool))
+    assert back.nelem < g.nelem and (units3[:, 0] == amr.UNIT_COARSEN).any()
+    assert back.level.min() >= g.minlevel
+    back.connectivity()  # still 2:1 balanced
+
+
+def test_adapt_refine_coarsen_inverse():
+    f = _forest(lmax=5)
+    flags = np.zeros(f.nelem, dtype=bool)
+    flags[f.level == f.level.max()] = True
+    g, units = f.adapt(flags, np.zeros(f.nelem, bool))
+    # coarsen exactly the families the refinement (and its balancing) made
+    c = np.zeros(g.nelem, bool)
+    for kind, io, jn in units:
+        if kind == amr.UNIT_REFINE:
+            c[jn:jn + 8] = True
+    assert c.sum() > 8 * flags.sum()  # balancing refined neighbours too
+    h, units2 = g.adapt(np.zeros(g.nelem, bool), c)
+    assert np.array_equal(h.level, f.level) and np.array_equal(h.anchor, f.anchor)
+    assert np.all(units2[:, 0] != amr.UNIT_REFINE)
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_mortar_ops_are_l2_projection(N):
+    M = N + 1
+    P, R = O.mortar_ops(N)
+    x, w = O.lgl(N)
+
+    def lag(xs, p):
+        v = np.ones_like(xs)
+        for q in range(M):
+            if q != p:
+                v = v * (xs - x[q]) / (x[p] - x[q])
+        return v
+
+    g, gw = np.polynomial.legendre.leggauss(M + 2)
+    mass = np.array([[np.sum(gw * lag(g, p) * lag(g, q)) for q in range(M)] for p in range(M)])
+    for c in (0, 1):
+        # interpolation: P applied to samples of a degree-N polynomial
+        coef = np.arange(1, M + 1) / 7.0
+        poly = lambda t: np.polyval(coef, t)
+        xs = 0.5 * (x + (2 * c - 1))
+        assert np.abs(P[c] @ poly(x) - poly(xs)).max() < 1e-14
+        xi = 0.5 * (g + (2 * c - 1))
+        B = np.array([[0.5 * np.sum(gw * lag(xi, p) * lag(g, i)) for i in range(M)] for p in range(M)])
+        assert np.abs(np.linalg.solve(mass, B) - R[c]).max() < 1e-13
+        assert np.abs(w @ R[c] - 0.5 * w).max() < 1e-15
+    assert np.abs(R[0] @ P[0] + R[1] @ P[1] - np.eye(M)).max() < 1e-13
+
+
+def _state_at_rest(f, N):
+    """rho, p polynomials of degree <= 2 at rest; exact RHS = (0, -grad p, 0)."""
+    M = N + 1
+    x, _ = O.lgl(N)
+    X = f.node_coords(x)
+    xx, yy, zz = X[:, 0] - 0.5, X[:, 1] - 0.5, X[:, 2] - 0.5
+    p = 1.0 + 0.3 * xx ** 2 + 0.2 * yy * zz + 0.1 * xx
+    u = np.zeros((f.nelem, 5, M ** 3))
+    u[:, 0] = 1.0 + 0.1 * yy + 0.05 * xx * zz
+    u[:, 4] = p / 0.4
+    ex = np.zeros_like(u)
+    ex[:, 1] = -(0.6 * xx + 0.1)
+    ex[:, 2] = -0.2 * zz
+    ex[:, 3] = -0.2 * yy
+    return u, ex
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+@pytest.mark.parametrize("vol,surf", [(O.VOL_CENTRAL, O.SURF_RUSANOV), (O.VOL_WEAK, O.SURF_HLLC)])
+def test_oracle_mortar_exact_state(N, vol, surf):
+    f = _forest()
+    cfg = O.with_mortars(O.make_cfg(N=N, volume=vol, surface=surf), f.mortars)
+    u, ex = _state_at_rest(f, N)
+    r = O.rhs(cfg, f.nbr, f.delta, u)
+    inner = np.all((f.lower > 1e-12) & (f.lower + f.delta < 1 - 1e-12), axis=1)  # periodic wrap breaks the polynomial
+    on_mortar = np.zeros(f.nelem, bool)
+    on_mortar[f.mortars[:, 0]] = True
+    on_mortar[f.mortars[:, 2:].ravel()] = True
+    assert (inner & on_mortar).sum() > 50
+    scale = np.abs(ex).max() * N / f.delta.min()
+    assert np.abs(r - ex)[inner].max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_oracle_mortar_free_stream_and_conservation(N):
+    f = _forest()
+    M = N + 1
+    cfg = O.with_mortars(O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP), f.mortars)
+    u = np.zeros((f.nelem, 5, M ** 3))
+    u[:, 0], u[:, 1], u[:, 2], u[:, 3], u[:, 4] = 1.3, 0.3, -0.2, 0.1, 2.5
+    r = O.rhs(cfg, f.nbr, f.delta, u)
+    assert np.abs(r).max() < 1e-12 * M / f.delta.min()
+    u = indicator_mlp.sedov(f, N, radius=0.05, p0=1e-2)
+    a, _ = O.indicator(cfg, u)
+    assert (a > 0).any()
+    r = O.rhs(cfg, f.nbr, f.delta, u)
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    tot = np.einsum("evn,n,e->v", r, w3, np.prod(f.delta, axis=1) / 8.0)
+    assert np.abs(tot).max() <= 1e-13 * np.abs(r).max()
+
+
+def _poly_state(f, N, shift=0.0):
+    M = N + 1
+    x, _ = O.lgl(N)
+    X = f.node_coords(x)
+    u = np.zeros((f.nelem, 5, M ** 3))
+    xx, yy, zz = X[:, 0], X[:, 1], X[:, 2]
+    u[:, 0] = 2.0 + 0.3 * xx - 0.2 * yy * zz + shift
+    u[:, 1] = 0.1 * xx * yy
+    u[:, 2] = -0.2 * zz
+    u[:, 3] = 0.05
+    u[:, 4] = 3.0 + 0.1 * xx * xx * zz
+    return u
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_oracle_remap_exact_and_conservative(N):
+    f = _forest(lmax=5)
+    flags = np.zeros(f.nelem, bool)
+    flags[::3] = True
+    g, units = f.adapt(flags, np.zeros(f.nelem, bool))
+    # refinement interpolates polynomials of degree <= N exactly
+    u = _poly_state(f, N)
+    ug, nfb = O.remap(N, 1.4, u, units, g.nelem)
+    assert nfb == 0
+    assert np.abs(ug - _poly_state(g, N)).max() < 1e-13
+    # coarsening back is the inverse of refinement (L2 projection)
+    c = np.zeros(g.nelem, bool)
+    for kind, io, jn in units:
+        if kind == amr.UNIT_REFINE:
+            c[jn:jn + 8] = True
+    h, units2 = g.adapt(np.zeros(g.nelem, bool), c)
+    assert np.array_equal(h.anchor, f.anchor)
+    rng = np.random.default_rng(1)
+    v = u * (1.0 + 0.01 * rng.random(u.shape))
+    vg, _ = O.remap(N, 1.4, v, units, g.nelem)
+    vh, _ = O.remap(N, 1.4, vg, units2, h.nelem)
+    assert np.abs(vh - v).max() < 1e-13
+    # a coarsening of non-polynomial data conserves every variable
+    wg = vg * (1.0 + 0.01 * rng.random(vg.shape))
+    wh, _ = O.remap(N, 1.4, wg, units2, h.nelem)
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    assert np.allclose(np.einsum("evn,n,e->v", wh, w3, np.prod(h.delta, 1)),
+                       np.einsum("evn,n,e->v", wg, w3, np.prod(g.delta, 1)), rtol=1e-14, atol=0)
+
+
+def test_oracle_remap_positivity_fallback():
+    N = 3
+    f = amr.Forest.uniform(1, maxlevel=2)
+    M = N + 1
+    x, w = O.lgl(N)
+    u = np.zeros((f.nelem, 5, M ** 3))
+    u[:, 0] = 1.0
+    # nodal pressure (1e-3, 1e-8, 1e-8, 1e-3) along x: positive at the nodes,
+    # its cubic interpolant is -2.5e-4 at the element centre, a child node
+    i = np.arange(M ** 3) % M
+    u[:, 4] = np.where((i == 0) | (i == M - 1), 1e-3, 1e-8)[None, :] / 0.4
+    assert all(O.isvalid(O.make_cfg(), u[0, :, n]) for n in range(M ** 3))
+    flags = np.ones(f.nelem, bool)
+    g, units = f.adapt(flags, np.zeros(f.nelem, bool))
+    ug, nfb = O.remap(N, 1.4, u, units, g.nelem)
+    assert nfb > 0
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    assert np.allclose(np.einsum("evn,n,e->v", ug, w3, np.prod(g.delta, 1)),
+                       np.einsum("evn,n,e->v", u, w3, np.prod(f.delta, 1)), rtol=1e-14, atol=1e-18)
+    for e in range(g.nelem):
+        rho = ug[e, 0]
+        p = 0.4 * (ug[e, 4] - 0.5 * (ug[e, 1:4] ** 2).sum(0) / rho)
+        assert (rho > 0).all() and (p > 0).all()

@@ -1,0 +1,210 @@
+"""Adaptive meshes on the GPU: non-conforming (2:1) faces and the state
+transfer of mesh adaptation, through the C-ABI, against the FP64 oracle.
+
+Bars as for conforming meshes: RHS per call <= 1e-12 relative L-inf,
+20 RK steps <= 1e-10; the state transfer (interpolation / projection with
+the positivity fallback) <= 1e-14 relative, the fallback count exact."""
+import numpy as np
+import pytest
+
+from nemo_amd import _lib, amr, solver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+RUN_TOL = 1e-10
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _forest(lmin=2, lmax=4):
+    f = amr.Forest.uniform(lmin, maxlevel=lmax)
+    c0 = np.array([0.45, 0.4, 0.5])
+    for r in (0.3, 0.15):
+        cen = f.lower + 0.5 * f.delta
+        f = f.refine(np.linalg.norm(cen - c0, axis=1) < r).balance()
+    return f
+
+
+def _state(f, N, gamma=1.4):
+    """Smooth periodic flow plus a narrow pressure bump (drives the indicator)."""
+    x, _ = O.lgl(N)
+    X = f.node_coords(x)
+    s = lambda t: np.sin(2 * np.pi * t)
+    c = lambda t: np.cos(2 * np.pi * t)
+    rho = 1.0 + 0.2 * s(X[:, 0]) * c(X[:, 1])
+    vx, vy, vz = 0.3 * s(X[:, 2]), 0.2 * c(X[:, 0]), 0.1 + 0.0 * X[:, 0]
+    r2 = ((X - np.array([0.45, 0.4, 0.5])[None, :, None]) ** 2).sum(1)
+    p = 1.0 + 0.3 * c(X[:, 0] + X[:, 1] + X[:, 2]) + 4.0 * np.exp(-r2 / (2 * 0.03 ** 2))
+    u = np.stack([rho, rho * vx, rho * vy, rho * vz, p / (gamma - 1) + 0.5 * rho * (vx ** 2 + vy ** 2 + vz ** 2)], 1)
+    return np.ascontiguousarray(u)
+
+
+def _cfgs(f, N, vol, surf, ind, td=_lib.TD_LSRK54, repair=0, precision=0):
+    g = solver.make_config(N=N, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair,
+                           precision=precision)
+    o = O.with_mortars(O.make_cfg(N=N, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair),
+                       f.mortars)
+    return g, o
+
+
+COMBOS = [
+    (3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP),
+    (3, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE),
+    (3, _lib.VOL_CENTRAL, _lib.SURF_HLLC, _lib.IND_NONE),
+    (2, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP),
+    (4, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP),
+    (4, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE),
+]
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=lambda c: f"N{c[0]}-vol{c[1]}-surf{c[2]}-ind{c[3]}")
+def test_amr_rhs_parity(combo):
+    N, vol, surf, ind = combo
+    f = _forest()
+    u0 = _state(f, N)
+    g, o = _cfgs(f, N, vol, surf, ind)
+    print(f"amr rhs parity {combo}: {f.nelem} elements, {len(f.mortars)} mortars", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(f)
+        s.set_state(u0)
+        r = s.kernel_calc_flux()
+        a = s.indicator()
+    aref = O.indicator(o, u0)[0] if ind == _lib.IND_PP else None
+    if ind == _lib.IND_PP:
+        print(f"  indicator max diff {np.abs(a - aref).max():.3e}")
+        assert np.abs(a - aref).max() <= 1e-11 and (aref > 0).any()
+    rref = O.rhs(o, f.nbr, f.delta, u0, aref)
+    err = _rel(r, rref)
+    print(f"  rel Linf {err:.3e}")
+    if ind == _lib.IND_PP:
+        print(f"  rel Linf with the device's blending factors {_rel(r, O.rhs(o, f.nbr, f.delta, u0, a)):.3e}")
+    assert err <= RHS_TOL
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_amr_run_parity_and_conservation(N):
+    f = _forest()
+    u0 = _state(f, N)
+    g, o = _cfgs(f, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, repair=1)
+    print(f"amr run parity N={N}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(f)
+        s.set_state(u0)
+        h = s.run(20, record_dt=True)
+        u = s.get_state()
+    uref = u0.copy()
+    rc, href = O.run(o, f.nbr, f.delta, uref, 20)
+    assert rc == 0
+    assert np.abs(h - href).max() <= 1e-12 * href.max()
+    err = _rel(u, uref)
+    print(f"  rel Linf after 20 steps {err:.3e}")
+    assert err <= RUN_TOL
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    vol = np.prod(f.delta, 1) / 8.0
+    m0 = np.einsum("evn,n,e->v", u0, w3, vol)
+    m1 = np.einsum("evn,n,e->v", u, w3, vol)
+    assert np.abs(m1[[0, 4]] - m0[[0, 4]]).max() <= 1e-13 * np.abs(m0[[0, 4]]).max()
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_remesh_matches_oracle(N):
+    c1 = np.array([0.3, 0.3, 0.3])
+    f = amr.Forest.uniform(3, minlevel=2, maxlevel=5)
+    f, _ = f.adapt(np.linalg.norm(f.lower + 0.5 * f.delta - c1, axis=1) < 0.2, np.zeros(f.nelem, bool))
+    u0 = _state(f, N)
+    M = N + 1
+    # one element whose nodal pressure is 1e-3 on the x = -1 face and 1e-8
+    # elsewhere: positive at its nodes, but the interpolant (1e-3 l_0(x))
+    # is negative at child nodes (x = 1/2 for N = 2, 0 for N = 3, -1/2 for
+    # N = 4): fallback to the element mean
+    i = np.arange(M ** 3) % M
+    e_bad = int(np.argmax(f.level))
+    u0[e_bad] = 0.0
+    u0[e_bad, 0] = 1.0
+    u0[e_bad, 4] = np.where(i == 0, 1e-3, 1e-8) / 0.4
+    d = np.linalg.norm(f.lower + 0.5 * f.delta - c1, axis=1)
+    ref = d < 0.1
+    ref[e_bad] = True
+    g, units = f.adapt(ref, d > 0.5)
+    assert units is not None
+    kinds = np.bincount(units[:, 0], minlength=3)
+    assert kinds[amr.UNIT_REFINE] > 0 and kinds[amr.UNIT_COARSEN] > 0
+    cfg, o = _cfgs(g, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, repair=1)
+    print(f"remesh N={N}: {f.nelem} -> {g.nelem} elements, units {kinds.tolist()}", flush=True)
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(f)
+        s.set_state(u0)
+        nfb = s.remesh(g, units)
+        ug = s.get_state()
+        r = s.kernel_calc_flux()
+    uref, nfb_ref = O.remap(N, 1.4, u0, units, g.nelem)
+    assert nfb == nfb_ref >= 1
+    assert _rel(ug, uref) <= 1e-14
+    # the new connectivity is in place: the RHS on the new mesh matches
+    assert _rel(r, O.rhs(o, g.nbr, g.delta, ug, O.indicator(o, ug)[0])) <= RHS_TOL
+
+
+def test_amr_fp32_operator():
+    N = 3
+    f = _forest()
+    u0 = _state(f, N)
+    g, o = _cfgs(f, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, precision=1)
+    with solver.Solver(g) as s:
+        s.upload_mesh(f)
+        s.set_state(u0)
+        r = s.kernel_calc_flux()
+    err = _rel(r, O.rhs(o, f.nbr, f.delta, u0, O.indicator(o, u0)[0]))
+    print(f"amr fp32 rhs rel Linf {err:.3e}")
+    assert err <= 5e-5
+
+
+def test_mesh_checks_reject_bad_mortars():
+    f = _forest()
+    g, _ = _cfgs(f, 3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP)
+    bad = f.mortars.copy()
+    bad[0, 2], bad[0, 3] = bad[0, 3], bad[0, 2]
+    with solver.Solver(g) as s:
+        class _M:
+            nbr, delta, mortars, nelem = f.nbr, f.delta, bad, f.nelem
+        with pytest.raises(_lib.NemoGpuError):
+            s.upload_mesh(_M)
+
+
+def test_adaptive_sedov_run_conserves_and_replays():
+    """Indicator-driven adaptation every 8 steps on a Sedov blast (levels
+    2-4): refinements and coarsenings happen, mass and energy are conserved
+    through them, and replaying the recorded meshes and time steps on a
+    second solver reproduces the state bit for bit."""
+    from nemo_amd import amr_run, indicator_mlp
+    N = 3
+    cfg = solver.make_config(N=N, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC, indicator=_lib.IND_PP,
+                             timedisc=_lib.TD_LSRK54, repair=1)
+    ic = lambda f: indicator_mlp.sedov(f, N, radius=0.04, p0=1e-3, seed=5)
+    f0 = amr_run.initial_forest(lambda: solver.Solver(cfg), amr.Forest.uniform(2, minlevel=2, maxlevel=4), ic)
+    assert f0.level.max() == 4 and len(f0.mortars) > 0
+    u0 = ic(f0)
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    mass = lambda f, u: np.einsum("evn,n,e->v", u, w3, np.prod(f.delta, 1) / 8.0)
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(f0)
+        s.set_state(u0)
+        run = amr_run.AdaptiveRun(s, f0, every=8)
+        run.advance(120, record=True)
+        u1 = s.get_state()
+        f1 = run.forest
+    print(f"elements per segment {run.elements}, remeshes {run.remeshes}, fallbacks {run.fallbacks}")
+    assert run.remeshes >= 3 and len(set(run.elements)) >= 3
+    m0, m1 = mass(f0, u0), mass(f1, u1)
+    assert np.abs(m1[[0, 4]] - m0[[0, 4]]).max() <= 1e-12 * np.abs(m0[[0, 4]]).max()
+    with solver.Solver(cfg) as s2:
+        s2.upload_mesh(f0)
+        s2.set_state(u0)
+        amr_run.replay(s2, run.history)
+        u2 = s2.get_state()
+    assert np.array_equal(u1, u2)

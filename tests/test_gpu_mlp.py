@@ -169,7 +169,10 @@ def test_trained_network_gain_on_sedov():
     undershoot of the plain indicator (min rho ~ 5e-2 instead of ~1e-3) and
     the plain indicator's solution (L1 density difference < 1e-4, against
     ~3e-3 for pure DG), while the Persson-Peraire indicator only runs on the
-    sampled 5% of elements per step."""
+    sampled 5% of elements per step.  At the final state fewer than 0.5% of
+    the elements have a prediction more than the judge's tolerance (0.05)
+    below the plain indicator (the judge corrects those only where it
+    samples them)."""
     params = indicator_mlp.load_params()
     for n, seed in ((16, 987), (24, 4321)):
         m = meshmod.uniform_periodic(n)
@@ -193,10 +196,11 @@ def test_trained_network_gain_on_sedov():
         assert isinstance(res["pp"], np.ndarray) and isinstance(res["mlp"], np.ndarray)
         d = np.abs(res["mlp"][:, 0] - res["pp"][:, 0]).sum() / np.abs(res["pp"][:, 0]).sum()
         a_pp, a_mlp = alphas["pp"], alphas["mlp"]
+        under = np.mean(a_pp - a_mlp > 0.05)
         print(f"  {n}^3 L1 density mlp vs pp {d:.3e}; blended pp {np.mean(a_pp > 0):.4f} mlp {np.mean(a_mlp > 0):.4f};"
-              f" max underestimate {np.max(a_pp - a_mlp):.3f}")
+              f" max underestimate {np.max(a_pp - a_mlp):.3f}, fraction under by > 0.05: {under:.5f}")
         assert d < 1e-4
-        assert np.max(a_pp - a_mlp) < 0.15
+        assert under < 0.005
         rmin = {k: float(res[k][:, 0].min()) for k in ("pp", "mlp")}
         assert rmin["mlp"] >= 0.9 * rmin["pp"]
         if isinstance(res["none"], np.ndarray):

@@ -241,6 +241,21 @@ def test_indicator_closed_forms():
         al = 1 / (1 + np.exp(-s / T * (E - T)))
         al = 0.0 if al < 0.001 else (1.0 if al > 0.999 else al)
         assert abs(a[0] - min(al, 0.5)) <= 1e-9
+    # energy in the second shell only (p = p0 + b Lhat_{N-1}(x) + c Lhat_N(y)
+    # with c small): Hennemann's second ratio second / (total - top) decides,
+    # E = max(top/total, second/(total - top))
+    for b, c in ((0.05, 0.0), (0.08, 0.01), (0.2, 0.002)):
+        LN1 = np.array([O.legendre(N - 1, xi)[0] for xi in x])
+        LN = np.array([O.legendre(N, xi)[0] for xi in x])
+        i = np.arange(M ** 3) % M
+        j = (np.arange(M ** 3) // M) % M
+        p = 1.0 + b * LN1[i] + c * LN[j]
+        u = np.ascontiguousarray(setups.prim2cons(np.ones((1, M ** 3)), 0 * p[None], 0 * p[None], 0 * p[None],
+                                                  p[None], GAMMA))
+        a, e = O.indicator(cfg, u)
+        e0, e1, e2 = (2 ** 1.5) ** 2, (2 * b) ** 2, (2 * c) ** 2
+        E = max(e2 / (e0 + e1 + e2), e1 / (e0 + e1))
+        assert abs(e[0] - E) <= 1e-12, (e[0], E)
 
 
 @pytest.mark.parametrize("timedisc", [O.TD_LSRK54, O.TD_SSPRK54])

@@ -246,19 +246,23 @@ def amr_sedov(steps: int, seed: int, lmin: int = 3, lmax: int = 6, every: int = 
             else:
                 history, f_end, u_end = run.history, run.forest, s.get_state()
             legs[name] = leg
-    with solver.Solver(cfg32) as s:
-        s.upload_mesh(f0)
-        s.set_state(u0)
-        s.mlp_set_params(params)
-        amr_run.replay(s, history)
-        u_rep = s.get_state()
     _, w = operators.lgl(N)
     w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
     vol = np.prod(f_end.delta, 1)[:, None] * w3[None, :] / 8.0
-    err = float(np.sum(np.abs(u_rep[:, 0] - u_end[:, 0]) * vol) / np.sum(np.abs(u_end[:, 0]) * vol))
     res.update(legs)
-    res["chosen_vs_fp64_same_meshes"] = {"l1_density_err": err, "l1_bar": 1e-6, "meets_l1_bar": err <= 1e-6,
-                                         "final_elements": f_end.nelem}
+    # replays of the FP64 + PP run's meshes and time steps: the chosen form,
+    # and each of its two departures alone (FP32 operator; learned indicator)
+    for name, cfg in (("chosen", cfg32), ("fp32_pp", solver.make_config(indicator=_lib.IND_PP, precision=1, **kw)),
+                      ("fp64_mlp", solver.make_config(indicator=_lib.IND_MLP, precision=0, **kw))):
+        with solver.Solver(cfg) as s:
+            s.upload_mesh(f0)
+            s.set_state(u0)
+            if cfg.indicator == _lib.IND_MLP:
+                s.mlp_set_params(params)
+            amr_run.replay(s, history)
+            u_rep = s.get_state()
+        err = float(np.sum(np.abs(u_rep[:, 0] - u_end[:, 0]) * vol) / np.sum(np.abs(u_end[:, 0]) * vol))
+        res[f"{name}_vs_fp64_same_meshes"] = {"l1_density_err": err, "final_elements": f_end.nelem}
     return res
 
 

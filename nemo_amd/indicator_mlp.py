@@ -87,13 +87,25 @@ def collect(runs, steps: int, every: int, precision: int = 1, log=print):
     return np.concatenate(feats), np.concatenate(alphas)
 
 
-def default_runs(n_sedov: int = 24, sedov_runs: int = 6, n_kh: int = 24, seed: int = 1234):
+def default_runs(n_sedov: int = 24, sedov_runs: int = 6, n_kh: int = 24, seed: int = 1234,
+                 amr_runs: int = 2, amr_seed: int = 2001, amr_levels=(3, 6)):
+    """Uniform Sedov blasts with jittered centres, one KH layer, and Sedov
+    blasts on the adaptive forests (levels 3-6) the indicator builds for
+    their initial data (blast centres other than the uniform runs')."""
+    from . import amr, amr_run
     runs = []
     m = meshmod.uniform_periodic(n_sedov)
     for k in range(sedov_runs):
         runs.append((m, sedov(m, 3, seed=seed + k)))
     mk = meshmod.uniform_periodic(n_kh)
     runs.append((mk, setups.kelvin_helmholtz(mk, 3, seed=seed)))
+    cfg = solver.make_config(indicator=_lib.IND_PP, precision=0, **SCHEME)
+    for k in range(amr_runs):
+        ic = lambda f, sd=amr_seed + k: sedov(f, 3, seed=sd)
+        f0 = amr_run.initial_forest(lambda: solver.Solver(cfg),
+                                    amr.Forest.uniform(amr_levels[0], minlevel=amr_levels[0],
+                                                       maxlevel=amr_levels[1]), ic)
+        runs.append((f0, ic(f0)))
     return runs
 
 

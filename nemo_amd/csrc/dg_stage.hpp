@@ -101,7 +101,7 @@ __device__ __forceinline__ void modal_partials(const StageArgs<M, Real>& a, cons
 }
 
 // Blending factor from the modal energy of Hennemann et al. (JCP 426, 2021,
-// eq. 48): max(top shell / total, second shell / energy below the top
+// sec. 3): max(top shell / total, second shell / energy below the top
 // shell); logistic ramp with sharpness s = log(9999) around the threshold
 // 0.5*10^(-1.8*M^0.25), values below alpha_min set to 0 (above 1-alpha_min
 // to 1), clipped at alpha_max.

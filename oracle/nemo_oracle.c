@@ -924,7 +924,7 @@ static double indicator_element(const orc_cfg *c, const orc_ops *op, const doubl
                 for (int j = 0; j < M; ++j) s += op->Vinv[b * M + j] * t1[a + M * j + M * M * k];
                 t2[a + M * b + M * M * k] = s;
             }
-    /* Hennemann et al. (JCP 426, 2021, eq. 48): energy = max(E_top / E,
+    /* Hennemann et al. (JCP 426, 2021, sec. 3): energy = max(E_top / E,
      * E_second / E_clip1) with E_clip1 the energy of the modes whose indices
      * are all below M-1, E_top = E - E_clip1 (largest index M-1) and
      * E_second = E_clip1 - E_clip2 (largest index M-2); the shells are summed

@@ -92,23 +92,6 @@ def _children(level: np.ndarray, anchor: np.ndarray):
     return lch, ach.reshape(-1, 3)
 
 
-def _containing_level(index: _Index, level_of: np.ndarray, point: np.ndarray, lmin: int, lmax: int) -> np.ndarray:
-    """Level of the leaf that contains each lattice point (-1 if none found)."""
-    out = np.full(len(point), -1, dtype=np.int64)
-    todo = np.ones(len(point), dtype=bool)
-    for lv in range(lmax, lmin - 1, -1):
-        if not todo.any():
-            break
-        mask = ~(np.int64(_size(np.array([lv]))[0]) - 1)
-        idx = np.nonzero(todo)[0]
-        a = point[idx] & mask
-        hit = index.find(np.full(len(idx), lv), a)
-        ok = hit >= 0
-        out[idx[ok]] = lv
-        todo[idx[ok]] = False
-    return out
-
-
 @dataclass
 class Forest:
     """Leaves of one periodic tree over the box [origin, origin + extent]."""
@@ -184,24 +167,24 @@ class Forest:
         return self._replace(level, anchor)
 
     def balance(self) -> "Forest":
-        """Refine until face neighbours differ by at most one level (2:1)."""
+        """Refine until face neighbours differ by at most one level (2:1).
+
+        The leaves tile the box in Morton order, so the leaf containing a
+        lattice point is the last one whose anchor key is not above the
+        point's key."""
         f = self
         while True:
-            index = _Index(f.level, f.anchor)
+            keys = morton_key(f.anchor[:, 0], f.anchor[:, 1], f.anchor[:, 2]).astype(np.int64)
             h = _size(f.level)
             need = np.zeros(f.nelem, dtype=bool)
-            lmin, lmax = int(f.level.min()), int(f.level.max())
             for d in range(3):
                 for s in (-1, 1):
                     p = f.anchor.copy()
                     p[:, d] = (p[:, d] + s * h) % ROOT
-                    lv = _containing_level(index, f.level, p, lmin, lmax)
-                    bad = (lv >= 0) & (lv < f.level - 1)
-                    if bad.any():
-                        mask = ~(_size(lv[bad]) - 1)
-                        a = p[bad] & mask[:, None]
-                        hit = index.find(lv[bad], a)
-                        need[hit[hit >= 0]] = True
+                    kp = morton_key(p[:, 0], p[:, 1], p[:, 2]).astype(np.int64)
+                    c = np.searchsorted(keys, kp, side="right") - 1
+                    bad = f.level[c] < f.level - 1
+                    need[c[bad]] = True
             if not need.any():
                 return f
             f = f.refine(need)
@@ -336,36 +319,44 @@ class Forest:
 def replacement_units(old: Forest, new: Forest) -> np.ndarray:
     """Pair old and new leaves (both in Morton order) into transfer units.
 
-    Walks both leaf sequences in lockstep like the replace loop of
-    mesh_adapt_wrapped (mesh_utils_mod.f90:967-1000): a new leaf one level
-    finer than the old one starts a refine unit (8 new), one level coarser a
-    coarsen unit (8 old), equal level and anchor a copy.
+    The pairing of the replace loop of mesh_adapt_wrapped
+    (mesh_utils_mod.f90:967-1000), which walks both leaf sequences in
+    lockstep: a new leaf one level finer than the old leaf containing its
+    anchor starts a refine unit (8 new), one level coarser a coarsen unit
+    (8 old), equal level a copy.  Computed at once from the Morton keys of
+    the anchors (a leaf covers the key range [key, key + 8^(bits - level))).
     """
-    units = []
-    i = j = 0
-    no, nn = old.nelem, new.nelem
-    while i < no and j < nn:
-        lo, ln = old.level[i], new.level[j]
-        if lo == ln:
-            if not np.array_equal(old.anchor[i], new.anchor[j]):
-                raise ValueError("old and new leaves out of step")
-            units.append((UNIT_COPY, i, j))
-            i += 1
-            j += 1
-        elif ln == lo + 1:
-            if not (np.all(new.level[j:j + 8] == ln) and np.array_equal(new.anchor[j], old.anchor[i])):
-                raise ValueError("refined leaf without its 8 children")
-            units.append((UNIT_REFINE, i, j))
-            i += 1
-            j += 8
-        elif ln == lo - 1:
-            if not (np.all(old.level[i:i + 8] == lo) and np.array_equal(new.anchor[j], old.anchor[i])):
-                raise ValueError("coarsened leaf without its 8 children")
-            units.append((UNIT_COARSEN, i, j))
-            i += 8
-            j += 1
-        else:
-            raise ValueError("a leaf changed by more than one level")
-    if i != no or j != nn:
+    ko = morton_key(old.anchor[:, 0], old.anchor[:, 1], old.anchor[:, 2]).astype(np.int64)
+    kn = morton_key(new.anchor[:, 0], new.anchor[:, 1], new.anchor[:, 2]).astype(np.int64)
+    if ko[0] != 0 or kn[0] != 0 or np.any(np.diff(ko) <= 0) or np.any(np.diff(kn) <= 0):
+        raise ValueError("leaves are not in Morton order")
+    io = np.searchsorted(ko, kn, side="right") - 1          # old leaf containing each new anchor
+    dl = new.level - old.level[io]
+    if np.any(np.abs(dl) > 1):
+        raise ValueError("a leaf changed by more than one level")
+    first = ko[io] == kn                                     # new anchor starts the old leaf
+    copy = (dl == 0) & first
+    refine = (dl == 1) & first
+    coarsen = dl == -1                                       # io is then the parent's first child
+    start = copy | refine | coarsen
+    if np.any((dl == 0) & ~first) or np.any((dl == -1) & ~first):
+        raise ValueError("old and new leaves out of step")
+    j = np.nonzero(start)[0]
+    kind = np.where(copy[j], UNIT_COPY, np.where(refine[j], UNIT_REFINE, UNIT_COARSEN))
+    units = np.stack([kind, io[j], j], axis=1).astype(np.int32)
+    # every old leaf consumed once, every new leaf produced once, in order
+    n_old = np.where(kind == UNIT_COARSEN, 8, 1)
+    n_new = np.where(kind == UNIT_REFINE, 8, 1)
+    if (not np.array_equal(units[:, 1], np.concatenate([[0], np.cumsum(n_old)[:-1]]))
+            or not np.array_equal(units[:, 2], np.concatenate([[0], np.cumsum(n_new)[:-1]]))
+            or n_old.sum() != old.nelem or n_new.sum() != new.nelem):
         raise ValueError("old and new leaves do not cover the same domain")
-    return np.asarray(units, dtype=np.int32).reshape(-1, 3)
+    ref = units[units[:, 0] == UNIT_REFINE]
+    for c in range(8):
+        if np.any(new.level[ref[:, 2] + c] != old.level[ref[:, 1]] + 1):
+            raise ValueError("refined leaf without its 8 children")
+    crs = units[units[:, 0] == UNIT_COARSEN]
+    for c in range(8):
+        if np.any(old.level[crs[:, 1] + c] != new.level[crs[:, 2]] + 1):
+            raise ValueError("coarsened leaf without its 8 children")
+    return units

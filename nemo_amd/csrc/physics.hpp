@@ -179,7 +179,15 @@ __device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& 
         s.p = g.kappa_m1 * (s.u[ENER] - Real(0.5) * s.inv_rho *
                                             (s.u[MOMX] * s.u[MOMX] + s.u[MOMY] * s.u[MOMY] + s.u[MOMZ] * s.u[MOMZ]));
     }
-    if constexpr (BETA) s.beta = fdiv(Real(0.5) * s.u[DENS], s.p);
+    // beta enters the log-mean as the ratio beta_a / beta_b, whose rounding
+    // the log branch amplifies by 1 / |beta_a / beta_b - 1| (up to 1e4 at the
+    // series switch): in FP64 it takes the reference's IEEE-rounded bits.
+    if constexpr (BETA) {
+        if constexpr (sizeof(Real) == 8)
+            s.beta = (Real(0.5) * s.u[DENS]) / s.p;
+        else
+            s.beta = fdiv(Real(0.5) * s.u[DENS], s.p);
+    }
 }
 
 // xflux (equations_mod.f90:64-79) from a node state.

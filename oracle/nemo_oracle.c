@@ -1051,22 +1051,32 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                         }
                     }
                 } else {
+                    /* each unordered pair once: f#(u_i, u_m) for i < m serves
+                     * both nodes (the flux is symmetric; evaluated in one
+                     * argument order, because the reference's log-mean
+                     * (a-b)/log(a/b) is not bitwise symmetric and its log
+                     * branch amplifies the rounding of a/b by 1/|a/b-1|) */
                     double acc[MAXM][MAXV];
                     for (int i = 0; i < M; ++i)
                         for (int h = 0; h < V; ++h) acc[i][h] = 0.0;
                     for (int i = 0; i < M; ++i)
-                        for (int m = 0; m < M; ++m) {
-                            if (m == i) continue;
-                            double F[MAXV];
+                        for (int m = i + 1; m < M; ++m) {
+                            double F[MAXV], Fi[MAXV], Fm[MAXV];
                             phys_two_point(c, c->volume, d, ul[i], ul[m], F);
+                            for (int h = 0; h < V; ++h) Fi[h] = Fm[h] = F[h];
                             if (mhd) {
                                 /* split form of the Powell/GLM terms:
                                  * sum_m S_im Phi(u_i) {{B_n, psi}}_im */
                                 double G[MAXV];
                                 phys_nc(c, d, ul[i], ul[i], ul[m], G);
-                                for (int h = 0; h < V; ++h) F[h] = F[h] + G[h];
+                                for (int h = 0; h < V; ++h) Fi[h] = F[h] + G[h];
+                                phys_nc(c, d, ul[m], ul[m], ul[i], G);
+                                for (int h = 0; h < V; ++h) Fm[h] = F[h] + G[h];
                             }
-                            for (int h = 0; h < V; ++h) acc[i][h] += op->S[i * M + m] * F[h];
+                            for (int h = 0; h < V; ++h) {
+                                acc[i][h] += op->S[i * M + m] * Fi[h];
+                                acc[m][h] += op->S[m * M + i] * Fm[h];
+                            }
                         }
                     for (int i = 0; i < M; ++i) {
                         int n = node_index(M, d, i, a, b);

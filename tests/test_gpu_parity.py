@@ -38,6 +38,9 @@ COMBOS = [
     (4, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),
     (2, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),
     (4, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),
+    # smooth data with many log-mean pairs just past the series switch
+    (3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "vortex"),
+    (4, _lib.VOL_EC, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_LSRK54, "vortex"),
 ]
 
 

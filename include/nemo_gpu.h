@@ -128,6 +128,15 @@ int nemo_gpu_rhs(void *handle, double *rhs);
 /* Blending factor per element for the current state (indicator only). */
 int nemo_gpu_indicator(void *handle, double *alpha);
 
+/* Volume integrals of the current state with the LGL quadrature volumes
+ * (the sums of hook_analyze, setups/euler/2d/_common_/setup_analyze_mod.f90:
+ * 34-164, and setups/euler/3d/convergence-test/setup_analyze_mod.f90:
+ * 40-151): sums[0..V-1] the conserved variables, [9] kinetic, [10] internal,
+ * [11] magnetic energy, [12] volume, [13] min rho, [14] min p, [15] max |v|.
+ * Deterministic (fixed reduction order).  Synchronises. */
+#define NEMO_ANALYZE_NSUMS 16
+int nemo_gpu_analyze(void *handle, double *sums);
+
 /* CFL time step of the current state (replaces hook_calc_timestep_200,
  * kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-63): dt =
  * 0.5*CFL/(2*N_NODES-1)*min(delta/lambda); lmax and vmax are the global

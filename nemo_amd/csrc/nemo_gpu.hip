@@ -16,6 +16,7 @@
 #include "../../include/nemo_gpu.h"
 #include "host_ops.hpp"
 #include "dg_mortar.hpp"
+#include "dg_analyze.hpp"
 
 namespace {
 
@@ -45,6 +46,9 @@ struct Solver {
     int32_t* mortars = nullptr;        // [nmortar][6]
     double* mortar_flux = nullptr;     // [5*nmortar][5][M*M]
     int* counter = nullptr;            // remap fallbacks
+    double* an_part = nullptr;         // analysis partials
+    long long an_cap = 0;
+    double* an_out = nullptr;          // [NEMO_ANALYZE_NSUMS]
     double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
     double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;
@@ -409,6 +413,23 @@ int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, 
 }
 
 template <int M>
+void launch_analyze(Solver& s, long long nblocks, bool mhd) {
+    nemo::AnalyzeArgs<M> a{};
+    a.u = s.u[s.cur];
+    a.delta = s.delta;
+    a.part = s.an_part;
+    a.nelem = s.nelem;
+    a.ph = make_phys64(s.cfg);
+    for (int i = 0; i < M; ++i) a.w[i] = s.ops.w[i];
+    if (mhd)
+        hipLaunchKernelGGL((nemo::analyze_kernel<nemo::EQ_MHD, M>), dim3(unsigned(nblocks)), dim3(nemo::AN_NT), 0,
+                           s.stream, a);
+    else
+        hipLaunchKernelGGL((nemo::analyze_kernel<nemo::EQ_EULER, M>), dim3(unsigned(nblocks)), dim3(nemo::AN_NT), 0,
+                           s.stream, a);
+}
+
+template <int M>
 void launch_remap(Solver& s, double* u_new, const int32_t* d_units, int64_t nunits, int64_t nelem_new) {
     nemo::RemapArgs<M> a{};
     a.u_old = s.u[s.cur];
@@ -516,6 +537,8 @@ int nemo_gpu_destroy(void* handle) {
     (void)hipFree(s->dt_dev);
     (void)hipFree(s->status);
     (void)hipFree(s->counter);
+    (void)hipFree(s->an_part);
+    (void)hipFree(s->an_out);
     (void)hipFree(s->mlp_p);
     (void)hipFree(s->mlp_inf);
     (void)hipFree(s->mlp_m);
@@ -652,6 +675,34 @@ int nemo_gpu_indicator(void* handle, double* alpha) {
     }
     if (int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, s->alpha)) return rc;
     HIP_TRY(hipMemcpyAsync(alpha, s->alpha, size_t(s->nelem) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_analyze(void* handle, double* sums) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!sums) return fail(NEMO_ERR_ARG, "null argument");
+    const long long nodes = s->nelem * (long long)s->M3;
+    const long long nblocks = (nodes + nemo::AN_NT - 1) / nemo::AN_NT;
+    if (nblocks > s->an_cap) {
+        (void)hipFree(s->an_part);
+        s->an_part = nullptr;
+        HIP_TRY(hipMalloc(&s->an_part, size_t(nblocks) * nemo::AN_NSUMS * sizeof(double)));
+        s->an_cap = nblocks;
+    }
+    if (!s->an_out) HIP_TRY(hipMalloc(&s->an_out, nemo::AN_NSUMS * sizeof(double)));
+    const bool mhd = s->cfg.equations == NEMO_EQ_GLM_MHD;
+    switch (s->M) {
+    case 3: launch_analyze<3>(*s, nblocks, mhd); break;
+    case 4: launch_analyze<4>(*s, nblocks, mhd); break;
+    case 5: launch_analyze<5>(*s, nblocks, mhd); break;
+    default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(nemo::analyze_finalize, dim3(1), dim3(nemo::AN_NT), 0, s->stream, s->an_part, nblocks, s->an_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(sums, s->an_out, nemo::AN_NSUMS * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return NEMO_OK;
 }

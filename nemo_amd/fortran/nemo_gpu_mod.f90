@@ -31,6 +31,7 @@ integer(c_int32_t), parameter :: NEMO_IND_NONE     = 0
 integer(c_int32_t), parameter :: NEMO_IND_PP       = 1
 integer(c_int32_t), parameter :: NEMO_IND_MLP      = 2
 integer(c_int32_t), parameter :: NEMO_MLP_NPARAMS  = 8705
+integer(c_int32_t), parameter :: NEMO_ANALYZE_NSUMS = 16
 integer(c_int32_t), parameter :: NEMO_TD_LSRK54    = 0
 integer(c_int32_t), parameter :: NEMO_TD_SSPRK54   = 1
 
@@ -98,6 +99,14 @@ interface
         integer(c_int32_t), intent(in)    :: nbr(*), mortars(*), units(*)
         real(c_double), intent(in)        :: delta(*)
         integer(c_int64_t), intent(out)   :: fallbacks
+        integer(c_int)                    :: rc
+    end function
+
+    !! volume integrals of hook_analyze: sums(NEMO_ANALYZE_NSUMS)
+    function nemo_gpu_analyze(handle, sums) result(rc) bind(c, name='nemo_gpu_analyze')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value                :: handle
+        real(c_double), intent(out)       :: sums(*)
         integer(c_int)                    :: rc
     end function
 

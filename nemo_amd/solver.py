@@ -150,6 +150,18 @@ class Solver:
         self._call("nemo_gpu_indicator", self._h, dptr(a))
         return a
 
+    def analyze(self) -> dict:
+        """Volume integrals and extrema of the current state (nemo_gpu_analyze)."""
+        out = np.empty(16)
+        self._call("nemo_gpu_analyze", self._h, dptr(out))
+        names = ["dens", "momx", "momy", "momz", "ener", "magx", "magy", "magz", "glmp"][:self.nvars]
+        res = {n: float(out[k]) for k, n in enumerate(names)}
+        res.update(ekin=float(out[9]), eint=float(out[10]), volume=float(out[12]), rho_min=float(out[13]),
+                   p_min=float(out[14]), v_max=float(out[15]))
+        if self.nvars == 9:
+            res["emag"] = float(out[11])
+        return res
+
     def hook_calc_timestep(self):
         """CFL time step; sets runtime.maxtimestep (kernel_timestep_mod.f90:19-63)."""
         dt, lm, vm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

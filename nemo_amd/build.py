@@ -68,6 +68,25 @@ def build_fortran_driver() -> str:
     return exe
 
 
+HDF5_PREFIX = os.environ.get("NEMO_HDF5_PREFIX", "/opt/conda")
+
+
+def build_flash() -> str | None:
+    """Host FLASH checkpoint writer libnemo_flash.so (needs the HDF5 C
+    library; skipped with a message where it is absent)."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 headers not found under {HDF5_PREFIX}: FLASH checkpoint writer not built")
+        return None
+    out = os.path.join(HERE, "libnemo_flash.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc, "-o", tmp,
+          os.path.join(HERE, "csrc", "flash_checkpoint.cpp"), "-L" + libdir, "-Wl,-rpath," + libdir, "-lhdf5"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference/source"):

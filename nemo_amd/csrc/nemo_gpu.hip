@@ -332,6 +332,17 @@ int status_check(Solver* s) {
     return NEMO_OK;
 }
 
+// Device allocation released when it goes out of scope unless handed over.
+struct DeviceBuffer {
+    void* p = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 // Mesh checks shared by the uploads: conforming neighbours in range, every
 // non-conforming face pointing at the one mortar slot that belongs to it.
 int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
@@ -369,10 +380,14 @@ int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double*
 
 // Device copies of a checked mesh; state-sized buffers are reallocated when
 // the element count changes (the state itself then needs an upload or a remap).
+// The solver counts as having no mesh until every copy is in place, so a
+// failure part way leaves no entry point running on half-built buffers.
 int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
              const int32_t* mortars) {
     HIP_TRY(hipStreamSynchronize(s.stream));
+    s.have_mesh = false;
     if (nelem != s.nelem) {
+        s.have_state = false;
         free_mesh_buffers(s);
         s.nelem = nelem;
         const size_t bytes = s.state_doubles() * sizeof(double);
@@ -563,8 +578,6 @@ int nemo_gpu_upload_mesh_amr(void* handle, int64_t nelem, const int32_t* nbr, co
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail(NEMO_ERR_ARG, "null handle");
     if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    if (nelem != s->nelem) s->have_state = false;
     return set_mesh(*s, nelem, nbr, delta, nmortar, mortars);
 }
 
@@ -591,34 +604,28 @@ int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const doubl
     if (io != s->nelem || jn != nelem) return fail(NEMO_ERR_ARG, "replacement units do not cover both meshes");
 
     const size_t new_bytes = size_t(s->V) * size_t(nelem) * size_t(s->M3) * sizeof(double);
-    double* u_new = nullptr;
-    int32_t* d_units = nullptr;
-    HIP_TRY(hipMalloc(&u_new, new_bytes));
-    HIP_TRY(hipMalloc(&d_units, size_t(nunits) * 3 * sizeof(int32_t)));
-    HIP_TRY(hipMemcpyAsync(d_units, units, size_t(nunits) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    DeviceBuffer u_new, d_units;  // freed on every early return
+    HIP_TRY(hipMalloc(&u_new.p, new_bytes));
+    HIP_TRY(hipMalloc(&d_units.p, size_t(nunits) * 3 * sizeof(int32_t)));
+    HIP_TRY(hipMemcpyAsync(d_units.p, units, size_t(nunits) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->counter, 0, sizeof(int), s->stream));
-    int rc = NEMO_OK;
+    double* un = static_cast<double*>(u_new.p);
+    const int32_t* du = static_cast<const int32_t*>(d_units.p);
     switch (s->M) {
-    case 3: launch_remap<3>(*s, u_new, d_units, nunits, nelem); break;
-    case 4: launch_remap<4>(*s, u_new, d_units, nunits, nelem); break;
-    case 5: launch_remap<5>(*s, u_new, d_units, nunits, nelem); break;
-    default: rc = fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    case 3: launch_remap<3>(*s, un, du, nunits, nelem); break;
+    case 4: launch_remap<4>(*s, un, du, nunits, nelem); break;
+    case 5: launch_remap<5>(*s, un, du, nunits, nelem); break;
+    default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
     }
-    if (rc == NEMO_OK) {
-        HIP_TRY(hipGetLastError());
-        int nfb = 0;
-        HIP_TRY(hipMemcpyAsync(&nfb, s->counter, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        if (fallbacks) *fallbacks = nfb;
-        rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars);
-    }
-    (void)hipFree(d_units);
-    if (rc != NEMO_OK) {
-        (void)hipFree(u_new);
-        return rc;
-    }
+    HIP_TRY(hipGetLastError());
+    int nfb = 0;
+    HIP_TRY(hipMemcpyAsync(&nfb, s->counter, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (fallbacks) *fallbacks = nfb;
+    if (int rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
     (void)hipFree(s->u[s->cur]);
-    s->u[s->cur] = u_new;
+    s->u[s->cur] = un;
+    u_new.p = nullptr;  // owned by the solver now
     HIP_TRY(hipMemsetAsync(s->r0, 0, new_bytes, s->stream));
     HIP_TRY(hipMemsetAsync(s->r1, 0, new_bytes, s->stream));
     HIP_TRY(hipMemsetAsync(s->alpha_step, 0, size_t(nelem) * sizeof(double), s->stream));

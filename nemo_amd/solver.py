@@ -52,6 +52,8 @@ class Runtime:
     rksimtime: float = 0.0
     rkstage: int = 0
     timesteps: int = 0
+    nextdump: float = 0.0
+    chkptcount: int = 0
     stat: Status = field(default_factory=Status)
 
 
@@ -265,20 +267,49 @@ class Solver:
         return n.value, ms.value
 
     # -- driver loop -------------------------------------------------------------
-    def mainloop(self, stop_time: float, max_steps: int = 1 << 30) -> Runtime:
-        """Time loop of driver_mod.f90:62-150 without checkpoint/analysis:
-        dt = min(maxtimestep, |stop_time - simtime|)."""
+    def mainloop(self, stop_time: float, max_steps: int = 1 << 30, *, dt_checkpoint: float = 0.0,
+                 checkpoint_dir: str = None, mesh=None, analysis=None, log=None) -> Runtime:
+        """Time loop of driver_mod.f90:62-150: CFL step, analysis
+        (hook_analyze, an AnalysisLog), a FLASH checkpoint of ``mesh`` into
+        ``checkpoint_dir`` whenever the time reaches the next dump time
+        (every ``dt_checkpoint``, and at the end), then the step with
+        dt = min(maxtimestep, dt_checkpoint, |nextdump - simtime|)."""
         rt = self.runtime
         tol = 4.0e-16
+        chk = checkpoint_dir is not None and dt_checkpoint > 0.0
+        rt.nextdump = rt.simtime if chk else float("inf")
         while rt.stat.ok and rt.timesteps < max_steps:
             self.hook_calc_timestep()
+            if analysis is not None:
+                analysis.record(self, rt.simtime, rt.maxtimestep, rt.timesteps)
+            if chk and abs(rt.simtime - rt.nextdump) < 10 * tol * max(1.0, abs(rt.simtime)):
+                self.hook_checkpoint(checkpoint_dir, mesh)
+                if log:
+                    log(f"[CHECKPOINT] {rt.chkptcount - 1:6d} {rt.timesteps:8d} {rt.simtime:12.3e} {rt.maxtimestep:12.4e}")
+                rt.nextdump = min(stop_time, rt.simtime + dt_checkpoint)
             if abs(rt.simtime - stop_time) < 100 * tol:
                 break
             rt.timestep = min(rt.maxtimestep, abs(stop_time - rt.simtime))
+            if chk:
+                rt.timestep = min(rt.timestep, dt_checkpoint, abs(rt.nextdump - rt.simtime))
             self.hook_evolve()
             rt.simtime += rt.timestep
             rt.timesteps += 1
         return rt
+
+    def hook_checkpoint(self, directory: str, mesh) -> str:
+        """FLASH checkpoint of the current state (hook_checkpoint_100..500,
+        checkpoint/flash/checkpoint_mod.f90:11-364) as
+        <directory>/chkpt_NNNNNN.f5; the blending factor goes to /blnd."""
+        from . import flash
+        rt = self.runtime
+        u = self.get_state()
+        blend = self.indicator() if self.cfg.indicator != _lib.IND_NONE else None
+        path = flash.write(flash.checkpoint_path(directory, rt.chkptcount), mesh, self.M - 1, u, time=rt.simtime,
+                           dt=rt.maxtimestep, nstep=rt.timesteps, number=rt.chkptcount, gamma=self.cfg.gamma,
+                           cfl=self.cfg.cfl, blend=blend)
+        rt.chkptcount += 1
+        return path
 
 
 class Batch:

@@ -1,5 +1,7 @@
 """Device volume integrals (nemo_gpu_analyze) against host quadrature sums
 and closed forms, on uniform and adaptive meshes, Euler and GLM-MHD."""
+import os
+
 import numpy as np
 import pytest
 
@@ -86,3 +88,37 @@ def test_integrals_conserved_by_run_and_log(tmp_path):
         assert abs(a1[k] - a0[k]) <= 1e-13 * max(1.0, abs(a0[k])), k
     rows = np.loadtxt(tmp_path / "analysis.dat")
     assert rows.shape == (2, len(analyze.AnalysisLog.COLUMNS))
+
+
+def test_mainloop_writes_checkpoints_and_analysis(tmp_path):
+    """driver_mod's loop with dt_checkpoint: a FLASH file at t = 0, at every
+    dump time and at the end, whose leaf data is the device state at that
+    time; the analysis log has one line per step."""
+    from nemo_amd import flash
+    N = 3
+    m = meshmod.uniform_periodic(4)
+    u = setups.kelvin_helmholtz(m, N, seed=3)
+    cfg = solver.make_config(N=N, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC, indicator=_lib.IND_PP,
+                             timedisc=_lib.TD_LSRK54, repair=1)
+    log = analyze.AnalysisLog(str(tmp_path / "analysis.dat"))
+    lines = []
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(m)
+        s.set_state(u)
+        rt = s.mainloop(0.01, dt_checkpoint=0.004, checkpoint_dir=str(tmp_path / "checkpoints"), mesh=m,
+                        analysis=log, log=lines.append)
+        u_end = s.get_state()
+    assert abs(rt.simtime - 0.01) < 1e-14 and rt.chkptcount == 4  # t = 0, 0.004, 0.008, 0.01
+    files = sorted(os.listdir(tmp_path / "checkpoints"))
+    assert files == [f"chkpt_{i:06d}.f5" for i in range(4)]
+    assert len(lines) == 4 and lines[0].startswith("[CHECKPOINT]")
+    rows = np.loadtxt(tmp_path / "analysis.dat")
+    assert rows.shape[0] == rt.timesteps + 1
+    import subprocess
+    h5dump = "/opt/conda/bin/h5dump"
+    if os.path.exists(h5dump):
+        out = str(tmp_path / "d.bin")
+        subprocess.run([h5dump, "-d", "/dens", "-b", "LE", "-o", out, str(tmp_path / "checkpoints" / files[-1])],
+                       check=True, capture_output=True)
+        d = np.fromfile(out, dtype="<f8").reshape(-1, 64)
+        assert np.array_equal(d[-m.nelem:], u_end[:, 0])

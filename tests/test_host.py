@@ -26,6 +26,23 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with the header"
 
 
+def test_flash_library_exports_its_header():
+    """libnemo_flash.so exports what include/nemo_flash.h declares, and the
+    ctypes mirrors of its structs have the C layout."""
+    import pytest
+    from nemo_amd import flash
+    if not os.path.exists(flash.LIB_PATH):
+        pytest.skip("libnemo_flash.so not built (no HDF5 C library)")
+    txt = open(os.path.join(ROOT, "include", "nemo_flash.h")).read()
+    names = sorted(set(re.findall(r"\bint\s+(nemo_flash_\w+)\s*\(", txt)))
+    assert names == ["nemo_flash_error_message", "nemo_flash_write"]
+    L = flash.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert ctypes.sizeof(flash.Meta) == 8 * 4 + 14 * 8
+    assert ctypes.sizeof(flash.Tree) == 2 * 8 + 8 * 8
+
+
 def test_config_struct_layout():
     assert ctypes.sizeof(_lib.Config) == 8 * 4 + 5 * 8
 

@@ -518,6 +518,17 @@ struct StageLds {
     static constexpr int DOUBLES = RAW_DOUBLES + (COMPACT ? NN : 0);
 };
 
+// LSRK 5-4 probe of one value (low-storage/4-5/timedisc_probe_mod.f90:31-37)
+// with the register value r0 already loaded: reg = f (stage 1) or
+// a_s r0 + f; u + c_s dt reg; r0 <- reg.
+template <int M, typename Real>
+__device__ __forceinline__ double probe_lsrk(const StageArgs<M, Real>& a, long long gi, int s, double dt, double u,
+                                             double r0, double rhs) {
+    const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * r0 + rhs;
+    a.r0[gi] = reg;
+    return u + a.rk.lsrk_c[s - 1] * dt * reg;
+}
+
 // One block of BE elements through a whole stage (the body of the stage
 // kernels); lds holds StageLds::DOUBLES doubles.
 template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP>
@@ -549,23 +560,30 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     }
 
     // ---- stage the block's state in LDS (coalesced) --------------------
-    if constexpr (COMPACT) {
-        for (int idx = tid; idx < NN; idx += NT) {
-            const int sl = slot<M>(idx);
-            double uu[5];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                uu[h] = idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
-                s_u[h][sl] = float(uu[h]);
-            }
-            s_p64[sl] = pressure(a.ph64.gas, uu);
-        }
-    } else {
+    // Every thread owns Q = NN / NT = M node slots per variable; all its
+    // loads are issued before the first is used, so the wavefront waits for
+    // HBM once, not once per node and variable.
+    static_assert(NN % NT == 0, "whole lines per thread");
+    constexpr int QS = NN / NT;
+    {
+        double uu[V][QS];
 #pragma unroll
         for (int h = 0; h < V; ++h)
-            for (int idx = tid; idx < NN; idx += NT)
-                s_u[h][slot<M>(idx)] =
-                    idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const int idx = tid + q * NT;
+                uu[h][q] = idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
+            }
+#pragma unroll
+        for (int q = 0; q < QS; ++q) {
+            const int sl = slot<M>(tid + q * NT);
+#pragma unroll
+            for (int h = 0; h < V; ++h) s_u[h][sl] = TU(uu[h][q]);
+            if constexpr (COMPACT) {
+                const double un[5] = {uu[0][q], uu[1][q], uu[2][q], uu[3][q], uu[4][q]};
+                s_p64[sl] = pressure(a.ph64.gas, un);
+            }
+        }
     }
     __syncthreads();
 
@@ -635,29 +653,65 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         }
         return st;
     };
+    // Each thread owns Q node slots per variable.  The loads of a variable
+    // (registers, and the FP64 state for the compact layout) are issued
+    // together before any is used, so a wavefront waits for memory once per
+    // variable instead of once per node.
+    constexpr int Q = (NN + NT - 1) / NT;
+    const bool lsrk = a.timedisc == TD_LSRK54;
     if constexpr (COMPACT) {
         // FP64 state from HBM, new state in registers until the operator's
-        // LDS is free (each thread owns M node slots per variable)
-        double st[V][M];
+        // LDS is free
+        double st[V][Q];
 #pragma unroll
-        for (int h = 0; h < V; ++h)
+        for (int h = 0; h < V; ++h) {
+            double uin[Q], reg[Q];
 #pragma unroll
-            for (int q = 0; q < M; ++q) {
+            for (int q = 0; q < Q; ++q) {
+                const int idx = tid + q * NT;
+                const long long gi = h * EM3 + e0 * M3 + idx;
+                const bool ok = idx < nvalid_nodes;
+                uin[q] = ok ? a.u_in[gi] : 0.0;
+                reg[q] = (ok && !(lsrk && s == 1)) ? a.r0[gi] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
                 const int idx = tid + q * NT;
                 st[h][q] = 0.0;
                 if (idx < nvalid_nodes) {
                     const long long gi = h * EM3 + e0 * M3 + idx;
-                    st[h][q] = probe(gi, a.u_in[gi], dudt(h, slot<M>(idx)));
+                    st[h][q] = lsrk ? probe_lsrk(a, gi, s, dt, uin[q], reg[q], dudt(h, slot<M>(idx)))
+                                    : probe(gi, uin[q], dudt(h, slot<M>(idx)));
                 }
             }
+        }
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < V; ++h)
 #pragma unroll
-            for (int q = 0; q < M; ++q) {
+            for (int q = 0; q < Q; ++q) {
                 const int idx = tid + q * NT;
                 if (idx < nvalid_nodes) s_u64[h][slot<M>(idx)] = st[h][q];
             }
+    } else if (lsrk) {
+#pragma unroll
+        for (int h = 0; h < V; ++h) {
+            double reg[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int idx = tid + q * NT;
+                reg[q] = (idx < nvalid_nodes && s != 1) ? a.r0[h * EM3 + e0 * M3 + idx] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int idx = tid + q * NT;
+                if (idx < nvalid_nodes) {
+                    const long long gi = h * EM3 + e0 * M3 + idx;
+                    const int sl = slot<M>(idx);
+                    s_u64[h][sl] = probe_lsrk(a, gi, s, dt, s_u64[h][sl], reg[q], dudt(h, sl));
+                }
+            }
+        }
     } else {
 #pragma unroll
         for (int h = 0; h < V; ++h)

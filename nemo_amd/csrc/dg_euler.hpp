@@ -14,6 +14,25 @@ __device__ __forceinline__ void riemann(const Gas<Real>& g, const NodeState<Real
         riemann_rusanov<D>(g, uL, uR, f);
 }
 
+// Neighbour ids and edge lengths of the thread's element, requested once per
+// stage together with the state, so that the direction passes can issue
+// their face loads without first waiting for the connectivity.
+struct ElemGeom {
+    int32_t nbr[6];
+    double dx[3];
+};
+
+template <int M, typename Real>
+__device__ __forceinline__ ElemGeom load_geom(const StageArgs<M, Real>& a, long long e) {
+    ElemGeom g;
+    const bool valid = e < a.nelem;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) g.nbr[f] = valid ? a.nbr[e * 6 + f] : 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g.dx[d] = valid ? a.delta[e * 3 + d] : 1.0;
+    return g;
+}
+
 // One direction pass: contributions of direction D to the node accumulators.
 // The thread's M line nodes (and the two neighbour face nodes) become node
 // states once; the surface Riemann problems, the M(M-1)/2 two-point fluxes
@@ -21,7 +40,7 @@ __device__ __forceinline__ void riemann(const Gas<Real>& g, const NodeState<Real
 template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
-                                               const double* s_p64) {
+                                               const double* s_p64, const ElemGeom& geo) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
     constexpr bool BETA = (VOL == VOL_EC);
@@ -55,9 +74,9 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         const double* mort_m = nullptr;
         const double* mort_p = nullptr;
         if (valid) {
-            const long long em = a.nbr[e * 6 + 2 * D];
-            const long long ep = a.nbr[e * 6 + 2 * D + 1];
-            const double dx = a.delta[e * 3 + D];
+            const long long em = geo.nbr[2 * D];
+            const long long ep = geo.nbr[2 * D + 1];
+            const double dx = geo.dx[D];
             jac = Real(2.0 / dx);
             sdx = Real(double(M) / dx);
             if (em < 0) mort_m = a.mortar_flux + size_t(-em - 1) * 5 * M2 + ta + M * tb;

@@ -559,6 +559,10 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         glm_alpha = a.glm[1];
     }
 
+    // connectivity of the thread's element, requested with the state
+    ElemGeom geo{};
+    if constexpr (EQ == EQ_EULER) geo = load_geom(a, e0 + tid / Sh::M2);
+
     // ---- stage the block's state in LDS (coalesced) --------------------
     // Every thread owns Q = NN / NT = M node slots per variable; all its
     // loads are issued before the first is used, so the wavefront waits for
@@ -601,11 +605,11 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         __syncthreads();
         direction_pass_mhd<2, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);
     } else {
-        direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+        direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
         __syncthreads();
-        direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+        direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
         __syncthreads();
-        direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64);
+        direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
     }
     __syncthreads();
 

@@ -112,11 +112,11 @@ __device__ __forceinline__ constexpr int nidx(int l, int a, int b) {
 }
 
 // LDS slot of node n = i + M*j + M*M*k inside an element.  For M = 4 the
-// slot is 16k + 4(i^j) + (i^k): a bijection under which the 64 lanes of any
-// direction pass (one node per line, 16 lines x 4 elements) read 16 distinct
-// 8-byte bank pairs, 4 lanes each -- the minimum of 4 LDS passes for a
-// 64-lane b64 access.  Without it the x- and y-passes hit only 4 bank pairs
-// (16-way conflicts).  Other M keep the natural order.
+// slot is 16k + 4(i^j) + (i^k): a bijection under which the 16 lines of one
+// element in any direction pass (one node per line) fall into 16 distinct
+// slots modulo 16, so one element's lanes never share a bank.  Without it
+// the x- and y-passes hit only 4 bank pairs (16-way conflicts).  Other M
+// keep the natural order.
 template <int M>
 __device__ __forceinline__ constexpr int swz(int n) {
     if constexpr (M == 4) {
@@ -127,11 +127,21 @@ __device__ __forceinline__ constexpr int swz(int n) {
     }
 }
 
+// LDS slot of node n of the block element whose slots start at base =
+// el*M^3.  (Spreading the four elements of a block over different bank
+// groups as well, by XORing 16*(el & 3) into the slot, removes the
+// remaining bank conflicts but measured slower: the extra address
+// arithmetic costs more than the conflicts.)
+template <int M>
+__device__ __forceinline__ constexpr int bslot(int base, int n) {
+    return base + swz<M>(n);
+}
+
 // LDS slot of block-local node index idx = element*M^3 + n.
 template <int M>
 __device__ __forceinline__ constexpr int slot(int idx) {
     constexpr int M3 = M * M * M;
-    return (idx / M3) * M3 + swz<M>(idx % M3);
+    return bslot<M>((idx / M3) * M3, idx % M3);
 }
 
 // Block index -> element block, so that each XCD (blocks are dealt to the 8

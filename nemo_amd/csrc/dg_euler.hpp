@@ -24,19 +24,166 @@ struct ElemGeom {
 
 template <int M, typename Real>
 __device__ __forceinline__ ElemGeom load_geom(const StageArgs<M, Real>& a, long long e) {
+    // Lanes past the last element take element 0's entries (their results
+    // are discarded): unconditional loads, so later loads need not wait.
     ElemGeom g;
-    const bool valid = e < a.nelem;
+    const long long ec = e < a.nelem ? e : 0;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) g.nbr[f] = valid ? a.nbr[e * 6 + f] : 0;
+    for (int f = 0; f < 6; ++f) g.nbr[f] = a.nbr[ec * 6 + f];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) g.dx[d] = valid ? a.delta[e * 3 + d] : 1.0;
+    for (int d = 0; d < 3; ++d) g.dx[d] = a.delta[ec * 3 + d];
     return g;
 }
 
-// One direction pass: contributions of direction D to the node accumulators.
-// The thread's M line nodes (and the two neighbour face nodes) become node
-// states once; the surface Riemann problems, the M(M-1)/2 two-point fluxes
-// and the M-1 FV interface fluxes all read those states.
+// Whether neighbour nb (a mortar slot when negative) is an element of the
+// block starting at e0.
+__device__ __forceinline__ bool in_block(long long nb, long long e0, long long nelem, int be) {
+    return nb >= e0 && nb < e0 + be && nb < nelem;
+}
+
+// Neighbour nodes across the two faces of the thread's line in each
+// direction, FP64 as stored.  They are requested right after the block's
+// state, so that their latency hides behind the LDS staging and the
+// indicator instead of stalling each direction pass.
+struct FaceNodes {
+    double m[3][5];  // across face 2D (x-, y-, z-)
+    double p[3][5];  // across face 2D+1 (x+, y+, z+)
+};
+
+template <int D, int M, int BE, typename Real>
+__device__ __forceinline__ void load_face_nodes(const StageArgs<M, Real>& a, long long e0, const ElemGeom& geo,
+                                                FaceNodes& fn) {
+    constexpr int M2 = M * M, M3 = M2 * M;
+    const int tid = threadIdx.x;
+    const int el = tid / M2;
+    const int line = tid - el * M2;
+    const int ta = line % M, tb = line / M;
+    const long long e = e0 + el;
+    const long long EM3 = a.nelem * (long long)M3;
+    // A face on a non-conforming interface takes its flux from the mortar
+    // pass: the Riemann solve then runs on the element's own face node and is
+    // discarded.  Lanes past the last element read element 0 (their results
+    // are discarded too); no branch, so the nodes stay in registers.
+    const bool valid = e < a.nelem;
+    const long long em = geo.nbr[2 * D];
+    const long long ep = geo.nbr[2 * D + 1];
+    const long long km = valid ? (em < 0 ? e : em) : 0;
+    const long long kp = valid ? (ep < 0 ? e : ep) : 0;
+    long long om = km * M3 + nidx<D, M>(em < 0 ? 0 : M - 1, ta, tb);
+    long long op = kp * M3 + nidx<D, M>(ep < 0 ? M - 1 : 0, ta, tb);
+    if constexpr (sizeof(Real) == 4) {
+        // FP32 operator: a neighbour inside the block is read from LDS by the
+        // surface pass, and its lanes all load one fixed address (a single
+        // cache line for all of them).  (In FP64 the extra LDS reads and
+        // selects cost more than the saved requests.)
+        if (in_block(em, e0, a.nelem, BE)) om = 0;
+        if (in_block(ep, e0, a.nelem, BE)) op = 0;
+    }
+    const double* um = a.u_in + om;
+    const double* up = a.u_in + op;
+#pragma unroll
+    for (int h = 0; h < 5; ++h) {
+        fn.m[D][h] = um[h * EM3];
+        fn.p[D][h] = up[h * EM3];
+    }
+}
+
+// Surface term of direction D: the two face Riemann problems of the thread's
+// line and their lifting to the end nodes, +jac f*_- / w_0 and
+// -jac f*_+ / w_{M-1}.  With LGL subcells the DG and FV surface terms are the
+// same, so the term does not depend on the blending factor and is added to
+// the accumulators before the volume passes (pass D = 0 initialises them).
+template <int D, int M, int BE, int SURF, typename Real, typename TU>
+__device__ __forceinline__ void surface_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
+                                             Real (*s_acc)[BE * M * M * M], const double* s_p64, const ElemGeom& geo,
+                                             const FaceNodes& fn) {
+    using Sh = StageShape<M, BE>;
+    constexpr int M2 = Sh::M2, M3 = Sh::M3;
+    constexpr bool F32 = sizeof(Real) == 4;
+    const int tid = threadIdx.x;
+    const int el = tid / M2;
+    const int line = tid - el * M2;
+    const int ta = line % M, tb = line / M;
+    const long long e = e0 + el;
+    const bool valid = e < a.nelem;
+    const Gas<Real>& g = a.gas;
+    const int base = el * M3;
+    const int n0 = bslot<M>(base, nidx<D, M>(0, ta, tb));
+    const int n1 = bslot<M>(base, nidx<D, M>(M - 1, ta, tb));
+
+    NodeState<Real> S0, S1, Nm, Np;
+#pragma unroll
+    for (int h = 0; h < 5; ++h) {
+        S0.u[h] = Real(s_u[h][n0]);
+        S1.u[h] = Real(s_u[h][n1]);
+        Nm.u[h] = Real(fn.m[D][h]);
+        Np.u[h] = Real(fn.p[D][h]);
+    }
+    if constexpr (F32) {
+        S0.p64 = s_p64[n0];
+        S1.p64 = s_p64[n1];
+        Nm.p64 = pressure(a.ph64.gas, fn.m[D]);
+        Np.p64 = pressure(a.ph64.gas, fn.p[D]);
+        // neighbours inside the block: their staged nodes in LDS
+        const long long em = geo.nbr[2 * D], ep = geo.nbr[2 * D + 1];
+        const bool im = valid && in_block(em, e0, a.nelem, BE);
+        const bool ip = valid && in_block(ep, e0, a.nelem, BE);
+        const int nm = bslot<M>((im ? int(em - e0) : 0) * M3, nidx<D, M>(M - 1, ta, tb));
+        const int np = bslot<M>((ip ? int(ep - e0) : 0) * M3, nidx<D, M>(0, ta, tb));
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            const Real lm = Real(s_u[h][nm]), lp = Real(s_u[h][np]);
+            Nm.u[h] = im ? lm : Nm.u[h];
+            Np.u[h] = ip ? lp : Np.u[h];
+        }
+        const double qm = s_p64[nm], qp = s_p64[np];
+        Nm.p64 = im ? qm : Nm.p64;
+        Np.p64 = ip ? qp : Np.p64;
+    }
+    make_state<D, false>(g, S0);
+    make_state<D, false>(g, S1);
+    make_state<D, false>(g, Nm);
+    make_state<D, false>(g, Np);
+    Real fm[5], fp[5];
+    riemann<SURF, D>(g, Nm, S0, fm);
+    riemann<SURF, D>(g, S1, Np, fp);
+    Real jac = Real(0);
+    if (valid) {
+        const long long em = geo.nbr[2 * D];
+        const long long ep = geo.nbr[2 * D + 1];
+        jac = Real(2.0 / geo.dx[D]);
+        if (em < 0) {
+            const double* mf = a.mortar_flux + size_t(-em - 1) * 5 * M2 + ta + M * tb;
+#pragma unroll
+            for (int h = 0; h < 5; ++h) fm[h] = Real(mf[h * M2]);
+        }
+        if (ep < 0) {
+            const double* mf = a.mortar_flux + size_t(-ep - 1) * 5 * M2 + ta + M * tb;
+#pragma unroll
+            for (int h = 0; h < 5; ++h) fp[h] = Real(mf[h * M2]);
+        }
+    }
+    const Real cm = jac * a.op.inv_w[0];
+    const Real cp = -jac * a.op.inv_w[M - 1];
+    if constexpr (D == 0) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
+#pragma unroll
+            for (int h = 0; h < 5; ++h) s_acc[h][n] = i == 0 ? cm * fm[h] : (i == M - 1 ? cp * fp[h] : Real(0));
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            s_acc[h][n0] += cm * fm[h];
+            s_acc[h][n1] += cp * fp[h];
+        }
+    }
+}
+
+// Volume pass of direction D: the thread's M line nodes become node states
+// once; the M(M-1)/2 two-point fluxes (or the weak-form derivative) and the
+// M-1 interior FV interface fluxes read those states.  Only LDS is read.
 template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
@@ -50,7 +197,6 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     const int ta = line % M, tb = line / M;
     const long long e = e0 + el;
     const bool valid = e < a.nelem;
-    const long long EM3 = a.nelem * (long long)M3;
     const Gas<Real>& g = a.gas;
     const int base = el * M3;
 
@@ -58,68 +204,25 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
     NodeState<Real> S[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
+        const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
 #pragma unroll
         for (int h = 0; h < 5; ++h) S[i].u[h] = Real(s_u[h][n]);
         make_state<D, BETA>(g, S[i]);
         if constexpr (F32) S[i].p64 = s_p64[n];
     }
     Real jac = Real(0), sdx = Real(0);
-    Real fm[5], fp[5];
-    {
-        NodeState<Real> Nm, Np;
-        // faces on a non-conforming interface take their flux from the
-        // mortar pass; the Riemann solve below then runs on the element's
-        // own face node and is discarded
-        const double* mort_m = nullptr;
-        const double* mort_p = nullptr;
-        if (valid) {
-            const long long em = geo.nbr[2 * D];
-            const long long ep = geo.nbr[2 * D + 1];
-            const double dx = geo.dx[D];
-            jac = Real(2.0 / dx);
-            sdx = Real(double(M) / dx);
-            if (em < 0) mort_m = a.mortar_flux + size_t(-em - 1) * 5 * M2 + ta + M * tb;
-            if (ep < 0) mort_p = a.mortar_flux + size_t(-ep - 1) * 5 * M2 + ta + M * tb;
-            const double* um = a.u_in + (em < 0 ? e : em) * M3 + nidx<D, M>(em < 0 ? 0 : M - 1, ta, tb);
-            const double* up = a.u_in + (ep < 0 ? e : ep) * M3 + nidx<D, M>(ep < 0 ? M - 1 : 0, ta, tb);
-            double dm[5], dp[5];
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                dm[h] = um[h * EM3];
-                dp[h] = up[h * EM3];
-                Nm.u[h] = Real(dm[h]);
-                Np.u[h] = Real(dp[h]);
-            }
-            if constexpr (F32) {
-                Nm.p64 = pressure(a.ph64.gas, dm);
-                Np.p64 = pressure(a.ph64.gas, dp);
-            }
-        } else {
-#pragma unroll
-            for (int h = 0; h < 5; ++h) {
-                Nm.u[h] = S[0].u[h];
-                Np.u[h] = S[M - 1].u[h];
-            }
-            Nm.p64 = S[0].p64;
-            Np.p64 = S[M - 1].p64;
-        }
-        make_state<D, false>(g, Nm);
-        make_state<D, false>(g, Np);
-        riemann<SURF, D>(g, Nm, S[0], fm);
-        riemann<SURF, D>(g, S[M - 1], Np, fp);
-        if (mort_m)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) fm[h] = Real(mort_m[h * M2]);
-        if (mort_p)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) fp[h] = Real(mort_p[h * M2]);
+    if (valid) {
+        const double dx = geo.dx[D];
+        jac = Real(2.0 / dx);
+        sdx = Real(double(M) / dx);
     }
 
     Real c[M][5];
     if constexpr (VOL == VOL_WEAK) {
         // 3-D LGL extension of kernel/dgsem/weakform/2d/kernel_flux_mod.f90:242-277:
-        // rhs_i = lagrM_i*(sdx f*_-) - lagrP_i*(sdx f*_+) + sum_j dgDiff_ij (sdx f_j)
+        // rhs_i = sum_j dgDiff_ij (sdx f_j) here; the surface lifting
+        // lagrM_i sdx f*_- - lagrP_i sdx f*_+ = jac (d_i0 f*_- - d_iN f*_+)/w_i
+        // is the surface pass's term.
         Real fx[M][5];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
@@ -134,15 +237,13 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
                 Real s = Real(0);
 #pragma unroll
                 for (int j = 0; j < M; ++j) s += a.op.dgDiff[i][j] * fx[j][h];
-                Real surf = Real(0);
-                if (i == 0) surf = a.op.lagrM[0] * (sdx * fm[h]);
-                if (i == M - 1) surf = surf - a.op.lagrP[M - 1] * (sdx * fp[h]);
-                c[i][h] = surf + s;
+                c[i][h] = s;
             }
     } else {
         // Split-form flux differencing (Gassner/Winters/Kopriva 2016):
-        // rhs_i = -jac [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ];
-        // each pair is evaluated once and used by both nodes (f# symmetric).
+        // rhs_i = -jac [ sum_m S_im f#(u_i,u_m) + (d_iN f*_+ - d_i0 f*_-)/w_i ],
+        // the bracket's surface part being the surface pass's term; each pair
+        // is evaluated once and used by both nodes (f# symmetric).
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
@@ -163,30 +264,27 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
                 }
             }
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            c[0][h] = c[0][h] - fm[h] * a.op.inv_w[0];
-            c[M - 1][h] = c[M - 1][h] + fp[h] * a.op.inv_w[M - 1];
-        }
-#pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
             for (int h = 0; h < 5; ++h) c[i][h] = -jac * c[i][h];
     }
 
     // FV subcells on the LGL sub-grid (Hennemann et al. 2021), first order:
-    // rhs_i = -jac (F_{i+1/2} - F_{i-1/2}) / w_i with the element-face
-    // fluxes f*_- and f*_+ at the ends; blended as (1-alpha) DG + alpha FV.
+    // rhs_i = -jac (F_{i+1/2} - F_{i-1/2}) / w_i, blended as
+    // (1-alpha) DG + alpha FV; the element-face fluxes F_{-1/2} = f*_- and
+    // F_{M-1/2} = f*_+ enter through the surface pass, whose term is the
+    // same for DG and FV.
     if (alpha > Real(0)) {
         const Real beta = Real(1) - alpha;
         Real Fprev[5];
 #pragma unroll
-        for (int h = 0; h < 5; ++h) Fprev[h] = fm[h];
+        for (int h = 0; h < 5; ++h) Fprev[h] = Real(0);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             Real Fnext[5];
             if (i == M - 1) {
 #pragma unroll
-                for (int h = 0; h < 5; ++h) Fnext[h] = fp[h];
+                for (int h = 0; h < 5; ++h) Fnext[h] = Real(0);
             } else {
                 riemann<SURF, D>(g, S[i], S[i + 1], Fnext);
             }
@@ -201,14 +299,9 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
 
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
+        const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            if constexpr (D == 0)
-                s_acc[h][n] = c[i][h];
-            else
-                s_acc[h][n] += c[i][h];
-        }
+        for (int h = 0; h < 5; ++h) s_acc[h][n] += c[i][h];
     }
 }
 

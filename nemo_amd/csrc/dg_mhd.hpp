@@ -44,7 +44,7 @@ __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, 
     MhdState<Real> S[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
+        const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
 #pragma unroll
         for (int h = 0; h < V; ++h) S[i].u[h] = Real(s_u[h][n]);
         mhd_make_state<BETA>(g, S[i]);
@@ -162,7 +162,7 @@ __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, 
 
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const int n = base + swz<M>(nidx<D, M>(i, ta, tb));
+        const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
 #pragma unroll
         for (int h = 0; h < V; ++h) {
             if constexpr (D == 0)

@@ -55,33 +55,33 @@ __device__ __forceinline__ void modal_partials(const StageArgs<M, Real>& a, cons
     {  // x: t1(a,j,k) = sum_i Vinv(a,i) q(i,j,k)
         Real v[M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) v[i] = q[base + swz<M>(i + M * ta + M2 * tb)];
+        for (int i = 0; i < M; ++i) v[i] = q[bslot<M>(base, i + M * ta + M2 * tb)];
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
 #pragma unroll
             for (int i = 0; i < M; ++i) s += a.op.Vinv[r][i] * v[i];
-            t1[base + swz<M>(r + M * ta + M2 * tb)] = s;
+            t1[bslot<M>(base, r + M * ta + M2 * tb)] = s;
         }
     }
     __syncthreads();
     {  // y: t2(i,b,k) = sum_j Vinv(b,j) t1(i,j,k)
         Real v[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) v[j] = t1[base + swz<M>(ta + M * j + M2 * tb)];
+        for (int j = 0; j < M; ++j) v[j] = t1[bslot<M>(base, ta + M * j + M2 * tb)];
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             Real s = Real(0);
 #pragma unroll
             for (int j = 0; j < M; ++j) s += a.op.Vinv[r][j] * v[j];
-            t2[base + swz<M>(ta + M * r + M2 * tb)] = s;
+            t2[bslot<M>(base, ta + M * r + M2 * tb)] = s;
         }
     }
     __syncthreads();
     {  // z: modal(a,b,c) = sum_k Vinv(c,k) t2(a,b,k); energies
         Real v[M];
 #pragma unroll
-        for (int k = 0; k < M; ++k) v[k] = t2[base + swz<M>(ta + M * tb + M2 * k)];
+        for (int k = 0; k < M; ++k) v[k] = t2[bslot<M>(base, ta + M * tb + M2 * k)];
         tot = c1 = s1 = sh2 = Real(0);
 #pragma unroll
         for (int r = 0; r < M; ++r) {
@@ -328,7 +328,7 @@ __device__ bool repair_element(const Phys64& P, const double* w, double* const* 
     constexpr int V = NVars<EQ>::V;
     auto node_ok = [&](int n) {
         Real uu[V];
-        for (int h = 0; h < V; ++h) uu[h] = su[h][base + swz<M>(n)];
+        for (int h = 0; h < V; ++h) uu[h] = su[h][bslot<M>(base, n)];
         return isvalid64<EQ>(P, uu);
     };
     auto all_ok = [&]() {
@@ -347,7 +347,7 @@ __device__ bool repair_element(const Phys64& P, const double* w, double* const* 
                 for (int i = 0; i < M; ++i) {
                     const Real ww = w[i] * w[j] * w[k];
                     wsum += ww;
-                    for (int h = 0; h < V; ++h) means[h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
+                    for (int h = 0; h < V; ++h) means[h] += ww * su[h][bslot<M>(base, i + M * j + M * M * k)];
                 }
         for (int h = 0; h < V; ++h) means[h] = means[h] / wsum;
         // squeeze all nodes towards the mean in repair_decr steps
@@ -359,7 +359,7 @@ __device__ bool repair_element(const Phys64& P, const double* w, double* const* 
             const Real f = squeeze / scale_prev;  // rescale current deviations
             for (int n = 0; n < M * M * M; ++n)
                 for (int h = 0; h < V; ++h) {
-                    Real* x = &su[h][base + swz<M>(n)];
+                    Real* x = &su[h][bslot<M>(base, n)];
                     *x = means[h] + (squeeze == Real(0) ? Real(0) : f) * (*x - means[h]);
                 }
             scale_prev = squeeze;
@@ -388,7 +388,7 @@ __device__ bool repair_element(const Phys64& P, const double* w, double* const* 
                                             const Real ww = w[i] * w[j] * w[k];
                                             tw[t] += ww;
                                             for (int h = 0; h < V; ++h)
-                                                tiles[t][h] += ww * su[h][base + swz<M>(i + M * j + M * M * k)];
+                                                tiles[t][h] += ww * su[h][bslot<M>(base, i + M * j + M * M * k)];
                                         }
                                 for (int h = 0; h < V; ++h) tiles[t][h] = tiles[t][h] / tw[t];
                             }
@@ -419,7 +419,7 @@ __device__ bool repair_element(const Phys64& P, const double* w, double* const* 
                         for (int j = 2 * p * nn; j < (2 * p + 2) * nn; ++j)
                             for (int i = 2 * q * nn; i < (2 * q + 2) * nn; ++i)
                                 for (int h = 0; h < V; ++h) {
-                                    Real* x = &su[h][base + swz<M>(i + M * j + M * M * k)];
+                                    Real* x = &su[h][bslot<M>(base, i + M * j + M * M * k)];
                                     *x = means[h] + squeeze * (*x - means[h]);
                                 }
                 }
@@ -445,7 +445,7 @@ __device__ __forceinline__ void block_cfl(const Phys64& P, const double* __restr
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, vm = 0.0;
     if (e0 + el < nelem) {
         for (int i = 0; i < M; ++i) {
-            const int n = el * M3 + swz<M>(i + M * line);
+            const int n = bslot<M>(el * M3, i + M * line);
             double uu[NVars<EQ>::V];
 #pragma unroll
             for (int h = 0; h < NVars<EQ>::V; ++h) uu[h] = s_u[h][n];
@@ -569,15 +569,25 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     // HBM once, not once per node and variable.
     static_assert(NN % NT == 0, "whole lines per thread");
     constexpr int QS = NN / NT;
+    FaceNodes fn;
     {
         double uu[V][QS];
 #pragma unroll
         for (int h = 0; h < V; ++h)
 #pragma unroll
             for (int q = 0; q < QS; ++q) {
+                // Slots past the last element take node 0 of element 0 (any
+                // valid state: their results are discarded).  An unconditional
+                // load, not a select of one: the compiler would sink that into
+                // a branch, after which every later load waits for it.
                 const int idx = tid + q * NT;
-                uu[h][q] = idx < nvalid_nodes ? a.u_in[h * EM3 + e0 * M3 + idx] : (h == DENS || h == ENER ? 1.0 : 0.0);
+                uu[h][q] = a.u_in[h * EM3 + (idx < nvalid_nodes ? e0 * M3 + idx : 0)];
             }
+        if constexpr (EQ == EQ_EULER) {
+            load_face_nodes<0, M, BE>(a, e0, geo, fn);
+            load_face_nodes<1, M, BE>(a, e0, geo, fn);
+            load_face_nodes<2, M, BE>(a, e0, geo, fn);
+        }
 #pragma unroll
         for (int q = 0; q < QS; ++q) {
             const int sl = slot<M>(tid + q * NT);
@@ -605,6 +615,12 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         __syncthreads();
         direction_pass_mhd<2, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);
     } else {
+        surface_pass<0, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
+        __syncthreads();
+        surface_pass<1, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
+        __syncthreads();
+        surface_pass<2, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
+        __syncthreads();
         direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
         __syncthreads();
         direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);

@@ -48,6 +48,27 @@ __device__ __forceinline__ double rcp_nr(double b) {
     e = fma(-b, r, 1.0);
     return fma(r, e, r);
 }
+// fma(a, b, c) for a constant c: the compiler otherwise copies every 64-bit
+// constant into a vector register pair (two v_mov per use, twenty per log
+// below) before a v_fmac; v_fma_f64 reads c from a scalar register pair,
+// set by s_mov on the scalar unit.  Same fused operation, same bits.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+// Correctly rounded a / b, the bits of the IEEE division, for operands that
+// keep b, 1/b and a/b normal and far from overflow (here: densities,
+// pressures and their ratios): y = 1/b to about 1 ulp, q = a y, and one
+// Markstein correction q + (a - b q) y (P. Markstein, IBM J. Res. Dev. 34,
+// 1990).  These are the correction steps of the compiler's division; its
+// v_div_scale range scaling, v_div_fixup special cases and the VCC-carried
+// v_div_fmas are left out, so independent divisions overlap.
+__device__ __forceinline__ double div_rn(double a, double b) {
+    const double y = rcp_nr(b);
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
 __device__ __forceinline__ double fdiv(double a, double b) { return a * rcp_nr(b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double frcp(double a) { return rcp_nr(a); }
@@ -71,7 +92,7 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(
 // carries double-double arithmetic for 0.5 ulp, three times the work):
 // x = m 2^k with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
 // s = (m-1)/(m+1), |s| <= 0.1716, summed as 2s + 2s^3 sum_j s^2j/(2j+3) up to
-// s^23 (truncation below 1e-17 relative; Abramowitz & Stegun 4.1.27), plus
+// s^21 (truncation below 1e-18 relative; Abramowitz & Stegun 4.1.27), plus
 // k log 2.  m - 1 is exact, so the result keeps its relative accuracy for x
 // near 1, where the log-mean divides by it.
 __device__ __forceinline__ double log_pos(double x) {
@@ -84,17 +105,23 @@ __device__ __forceinline__ double log_pos(double x) {
     const double f = m - 1.0;
     const double s = f * rcp_nr(m + 1.0);
     const double t = s * s;
-    double P = 1.0 / 23.0;
-    P = fma(P, t, 1.0 / 21.0);
-    P = fma(P, t, 1.0 / 19.0);
-    P = fma(P, t, 1.0 / 17.0);
-    P = fma(P, t, 1.0 / 15.0);
-    P = fma(P, t, 1.0 / 13.0);
-    P = fma(P, t, 1.0 / 11.0);
-    P = fma(P, t, 1.0 / 9.0);
-    P = fma(P, t, 1.0 / 7.0);
-    P = fma(P, t, 1.0 / 5.0);
-    P = fma(P, t, 1.0 / 3.0);
+    double P;
+    {
+        // P = 1/3 + t/5 + ... + t^8/19 + t^9/21 in Horner form; its first
+        // step is unfused because two constants cannot share one fma (one
+        // scalar operand per instruction).  The dropped t^10/23 term is
+        // below 1e-18 relative.
+#pragma clang fp contract(off)
+        P = t * (1.0 / 21.0) + 1.0 / 19.0;
+    }
+    P = fma_sc(P, t, 1.0 / 17.0);
+    P = fma_sc(P, t, 1.0 / 15.0);
+    P = fma_sc(P, t, 1.0 / 13.0);
+    P = fma_sc(P, t, 1.0 / 11.0);
+    P = fma_sc(P, t, 1.0 / 9.0);
+    P = fma_sc(P, t, 1.0 / 7.0);
+    P = fma_sc(P, t, 1.0 / 5.0);
+    P = fma_sc(P, t, 1.0 / 3.0);
     const double lm = fma(2.0 * s * t, P, 2.0 * s);
     return fma(double(k), 0.69314718055994530942, lm);
 }
@@ -114,6 +141,14 @@ template <typename Real>
 __device__ __forceinline__ Real pressure_ordered(const Gas<Real>& g, Real rho, Real m0, Real m1, Real m2, Real E) {
 #pragma clang fp contract(off)
     return g.kappa_m1 * (E - Real(0.5) / rho * (m0 * m0 + m1 * m1 + m2 * m2));
+}
+
+// The same pressure for node states of valid (positive-density) data, with
+// the division as div_rn: identical bits.
+__device__ __forceinline__ double pressure_ordered_rn(const Gas<double>& g, double rho, double m0, double m1, double m2,
+                                                      double E) {
+#pragma clang fp contract(off)
+    return g.kappa_m1 * (E - div_rn(0.5, rho) * (m0 * m0 + m1 * m1 + m2 * m2));
 }
 
 // Unrotated (x-ordered) pressure: isvalid, node primitives, indicator.
@@ -174,7 +209,8 @@ __device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& 
     s.v[1] = s.u[MOMY] * s.inv_rho;
     s.v[2] = s.u[MOMZ] * s.inv_rho;
     if constexpr (sizeof(Real) == 8) {
-        s.p = pressure_dir<D>(g, s.u);
+        s.p = pressure_ordered_rn(g, s.u[DENS], s.u[Mom<D, 0>::idx], s.u[Mom<D, 1>::idx], s.u[Mom<D, 2>::idx],
+                                  s.u[ENER]);
     } else {
         s.p = g.kappa_m1 * (s.u[ENER] - Real(0.5) * s.inv_rho *
                                             (s.u[MOMX] * s.u[MOMX] + s.u[MOMY] * s.u[MOMY] + s.u[MOMZ] * s.u[MOMZ]));
@@ -184,7 +220,7 @@ __device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& 
     // series switch): in FP64 it takes the reference's IEEE-rounded bits.
     if constexpr (BETA) {
         if constexpr (sizeof(Real) == 8)
-            s.beta = (Real(0.5) * s.u[DENS]) / s.p;
+            s.beta = div_rn(Real(0.5) * s.u[DENS], s.p);
         else
             s.beta = fdiv(Real(0.5) * s.u[DENS], s.p);
     }
@@ -287,17 +323,24 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
 // returned together with its reciprocal so that the flux needs no further
 // division by it: 1/L = log(a/b)/(a-b) on the log branch.
 __device__ __forceinline__ void logmean2(double a, double b, double& lm, double& inv_lm) {
-    const double x = a / b;  // IEEE: log(x) amplifies its rounding by 1/|x-1|
-    if (fabs(a - b) < 1e-4) {
-        const double c1 = 1.0 / 6.0, c2 = 2.0 / 45.0, c3 = 22.0 / 945.0;
-        const double u = (x * (x - 2.0) + 1.0) * rcp_nr(x * (x + 2.0) + 1.0);
-        lm = (a + b) * (0.5 - u * (c1 - u * (c2 - c3 * u)));
-        inv_lm = rcp_nr(lm);
-    } else {
-        const double l = log_pos(x);
-        lm = (a - b) * rcp_nr(l);
-        inv_lm = l * rcp_nr(a - b);
+    // Both branches are evaluated and selected: straight-line code lets the
+    // scheduler interleave the independent log-means of a line, where
+    // per-pair branches serialise them.
+    const double x = div_rn(a, b);  // IEEE bits: log(x) amplifies its rounding by 1/|x-1|
+    const double d = a - b;
+    const bool series = fabs(d) < 1e-4;
+    const double c1 = 1.0 / 6.0, c2 = 2.0 / 45.0, c3 = 22.0 / 945.0;
+    const double u = (x * (x - 2.0) + 1.0) * rcp_nr(x * (x + 2.0) + 1.0);
+    double w;
+    {
+#pragma clang fp contract(off)
+        w = c2 - c3 * u;
     }
+    const double ls = (a + b) * (0.5 - u * fma_sc(-u, w, c1));
+    const double l = log_pos(x);
+    const double r = rcp_nr(series ? ls : d);
+    lm = series ? ls : d * rcp_nr(l);
+    inv_lm = series ? r : l * r;
 }
 
 // FP32 log-mean.  The reference's log branch loses ~eps/|a/b-1| relative

@@ -599,6 +599,24 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
             }
         }
     }
+    // FP64 Euler with the two-point volume flux, M <= 4: the LSRK register
+    // of the RK update is requested here, so that it arrives during the
+    // operator instead of being waited for at the end.  Its 40 VGPRs fit
+    // under the 256 of two waves per SIMD there (not for the other volume
+    // forms or M = 5).  Unconditional loads -- stage 1 and the SSP scheme
+    // read u_in instead and ignore it -- keep the wait counts exact.
+    constexpr bool R0_EARLY = EQ == EQ_EULER && !F32 && VOL == VOL_EC && M <= 4;
+    double r0e[R0_EARLY ? V : 1][R0_EARLY ? QS : 1];
+    if constexpr (R0_EARLY) {
+        const double* rsrc = (a.timedisc == TD_LSRK54 && a.stage != 1) ? a.r0 : a.u_in;
+#pragma unroll
+        for (int h = 0; h < V; ++h)
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const int idx = tid + q * NT;
+                r0e[h][q] = rsrc[h * EM3 + (idx < nvalid_nodes ? e0 * M3 + idx : 0)];
+            }
+    }
     __syncthreads();
 
     Real alpha = Real(0);
@@ -720,7 +738,10 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int idx = tid + q * NT;
-                reg[q] = (idx < nvalid_nodes && s != 1) ? a.r0[h * EM3 + e0 * M3 + idx] : 0.0;
+                if constexpr (R0_EARLY)
+                    reg[q] = r0e[h][q];
+                else
+                    reg[q] = (idx < nvalid_nodes && s != 1) ? a.r0[h * EM3 + e0 * M3 + idx] : 0.0;
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {

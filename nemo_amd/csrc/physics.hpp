@@ -95,13 +95,19 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(
 // s = (m-1)/(m+1), |s| <= 0.1716, summed as 2s + 2s^3 sum_j s^2j/(2j+3) up to
 // s^21 (truncation below 1e-18 relative; Abramowitz & Stegun 4.1.27), plus
 // k log 2.  m - 1 is exact, so the result keeps its relative accuracy for x
-// near 1, where the log-mean divides by it.
-__device__ __forceinline__ double log_pos(double x) {
+// near 1, where the log-mean divides by it.  The three-argument form also
+// returns t = s^2; with unscaled set it takes m = x, k = 0 (then t is
+// ((x-1)/(x+1))^2 for any x, and the log is valid only for x near 1).
+__device__ __forceinline__ double log_pos(double x, bool unscaled, double& t_out) {
     int k = __builtin_amdgcn_frexp_exp(x);
     double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
     if (m < 0.70710678118654752440) {
         m = m + m;
         k = k - 1;
+    }
+    if (unscaled) {
+        m = x;
+        k = 0;
     }
     const double f = m - 1.0;
     const double s = f * rcp_nr(m + 1.0);
@@ -124,7 +130,12 @@ __device__ __forceinline__ double log_pos(double x) {
     P = fma_sc(P, t, 1.0 / 5.0);
     P = fma_sc(P, t, 1.0 / 3.0);
     const double lm = fma(2.0 * s * t, P, 2.0 * s);
+    t_out = t;
     return fma(double(k), 0.69314718055994530942, lm);
+}
+__device__ __forceinline__ double log_pos(double x) {
+    double t;
+    return log_pos(x, false, t);
 }
 __device__ __forceinline__ double flog(double a) { return log_pos(a); }
 __device__ __forceinline__ float flog(float a) { return __builtin_amdgcn_logf(a) * 0.69314718055994530942f; }
@@ -331,14 +342,19 @@ __device__ __forceinline__ void logmean2(double a, double b, double& lm, double&
     const double d = a - b;
     const bool series = fabs(d) < 1e-4;
     const double c1 = 1.0 / 6.0, c2 = 2.0 / 45.0, c3 = 22.0 / 945.0;
-    const double u = (x * (x - 2.0) + 1.0) * rcp_nr(x * (x + 2.0) + 1.0);
+    // The series branch's u = (x(x-2)+1)/(x(x+2)+1) = ((x-1)/(x+1))^2 is the
+    // t of the log's reduction taken without 2^k scaling, which the series
+    // lanes select (their log is not used).  Where |a-b| < 1e-4 and a, b are
+    // O(1), u < 1e-8 and enters only as u/6 against 1/2: the difference
+    // between the two roundings of it is below 1e-16 of the log-mean.
+    double u;
+    const double l = log_pos(x, series, u);
     double w;
     {
 #pragma clang fp contract(off)
         w = c2 - c3 * u;
     }
     const double ls = (a + b) * (0.5 - u * fma_sc(-u, w, c1));
-    const double l = log_pos(x);
     const double r = rcp_nr(series ? ls : d);
     lm = series ? ls : d * rcp_nr(l);
     inv_lm = series ? r : l * r;

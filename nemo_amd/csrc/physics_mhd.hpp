@@ -166,18 +166,21 @@ __device__ __forceinline__ void mhd_nc_add(const MhdGas<Real>& g, const MhdState
 
 // LN_MEAN: equations/mhd/two_point/ec/two_point_flux_mod.f90:203-219, with
 // its reciprocal.  FP32: the Ismail-Roe series form of logmean2.
+// Both branches are evaluated and selected (straight-line code, as the
+// Euler log-mean); Xi takes the IEEE-rounded bits (div_rn), which the log
+// branch amplifies by 1/|Xi - 1|.
 __device__ __forceinline__ void ln_mean(double aL, double aR, double& lm, double& inv_lm) {
-    const double Xi = aR / aL;
+    const double Xi = div_rn(aR, aL);
     const double u = (Xi * (Xi - 2.0) + 1.0) * rcp_nr(Xi * (Xi + 2.0) + 1.0);
-    if (u < 1.0e-4) {
-        const double d = 105.0 + u * (35.0 + u * (21.0 + u * 15.0));
-        lm = (aL + aR) * 52.5 * rcp_nr(d);
-        inv_lm = d * rcp_nr((aL + aR) * 52.5);
-    } else {
-        const double l = log_pos(Xi);
-        lm = (aR - aL) * rcp_nr(l);
-        inv_lm = l * rcp_nr(aR - aL);
-    }
+    const bool series = u < 1.0e-4;
+    const double d = 105.0 + u * (35.0 + u * (21.0 + u * 15.0));
+    const double s = (aL + aR) * 52.5;
+    const double l = log_pos(Xi);
+    const double dif = aR - aL;
+    // series: lm = s/d, 1/lm = d/s; log: lm = dif/l, 1/lm = l/dif
+    const double num = series ? s : dif, den = series ? d : l;
+    lm = num * rcp_nr(den);
+    inv_lm = den * rcp_nr(num);
 }
 __device__ __forceinline__ void ln_mean(float aL, float aR, float& lm, float& inv_lm) { logmean2(aL, aR, lm, inv_lm); }
 

@@ -367,19 +367,19 @@ __device__ __forceinline__ void logmean2(double a, double b, double& lm, double&
 // f^2 < 0.1 (truncation < 1e-9), and the log form beyond, where
 // |log(a/b)| > 0.65.
 __device__ __forceinline__ void logmean2(float a, float b, float& lm, float& inv_lm) {
+    // both forms evaluated and selected (straight-line code, as in FP64)
     const float r = __builtin_amdgcn_rcpf(a + b);
-    const float f = (a - b) * r;
+    const float d = a - b;
+    const float f = d * r;
     const float u = f * f;
-    if (u < 0.1f) {
-        const float F = 1.0f + u * (1.0f / 3.0f + u * (1.0f / 5.0f + u * (1.0f / 7.0f + u * (1.0f / 9.0f +
-                        u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f)))))));
-        lm = 0.5f * (a + b) * __builtin_amdgcn_rcpf(F);
-        inv_lm = 2.0f * F * r;
-    } else {
-        const float l = flog(a * __builtin_amdgcn_rcpf(b));
-        lm = (a - b) * __builtin_amdgcn_rcpf(l);
-        inv_lm = l * __builtin_amdgcn_rcpf(a - b);
-    }
+    const bool series = u < 0.1f;
+    const float F = 1.0f + u * (1.0f / 3.0f + u * (1.0f / 5.0f + u * (1.0f / 7.0f + u * (1.0f / 9.0f +
+                    u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f)))))));
+    const float l = flog(a * __builtin_amdgcn_rcpf(b));
+    const float num = series ? 0.5f * (a + b) : d;
+    const float den = series ? F : l;
+    lm = num * __builtin_amdgcn_rcpf(den);
+    inv_lm = series ? 2.0f * F * r : l * __builtin_amdgcn_rcpf(d);
 }
 
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:

@@ -127,11 +127,20 @@ template <int M, typename Real>
 void launch_group_stage(Batch* b, Group& gr, int stage, bool cfl) {
     constexpr int BE = Tile<M>::BE;
     const auto a = batch_args<M, Real>(b, gr, stage, cfl);
-    long long grid = b->grid_per_cu > 0 ? (long long)b->grid_per_cu * b->ncu : gr.nblocks;
+    if (b->grid_per_cu == 0) {
+        // one workgroup per block: the plain stage kernel (its blocks are
+        // dealt to the XCDs in contiguous stretches).  The persistent loop
+        // keeps per-thread index arithmetic live across blocks, which costs
+        // it about 120 VGPRs and most of its occupancy.
+        hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real>),
+                           dim3(unsigned(gr.nblocks)), dim3(BE * M * M), 0, b->stream, a);
+        return;
+    }
+    long long grid = (long long)b->grid_per_cu * b->ncu;
     if (grid > gr.nblocks) grid = gr.nblocks;
-    if (b->grid_per_cu > 0 && b->xcd) grid = (grid + 7) / 8 * 8;  // whole XCD rounds
+    if (b->xcd) grid = (grid + 7) / 8 * 8;  // whole XCD rounds
     hipLaunchKernelGGL((nemo::stage_kernel_persistent<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real>),
-                       dim3(unsigned(grid)), dim3(BE * M * M), 0, b->stream, a, b->grid_per_cu > 0 ? b->xcd : 0);
+                       dim3(unsigned(grid)), dim3(BE * M * M), 0, b->stream, a, b->xcd);
 }
 
 template <int M>

@@ -273,7 +273,42 @@ class Forest:
         return self.connectivity()[1]
 
     # ---- adaptation --------------------------------------------------------
-    def adapt(self, refine: np.ndarray, coarsen: np.ndarray):
+    def _drop_rerefined(self, starts: np.ndarray, refine: np.ndarray) -> np.ndarray:
+        """Remove the families whose parent balance would split again at once.
+
+        The parent P (level L) of a coarsened family is refined back into
+        the same 8 children when a face neighbour of P ends up at level
+        L + 2: a conforming neighbour of a child that is refined now, or a
+        finer neighbour (mortar big side) not all of whose four small
+        elements are coarsened now.  A coarser leaf never makes another leaf
+        refine, so leaving such a family alone gives the same forest; it
+        only skips building and balancing one that is undone."""
+        nbr, mortars = self.connectivity()
+        starts = starts.copy()
+        while True:
+            idx = np.nonzero(starts)[0]
+            if len(idx) == 0:
+                return starts
+            coarsened = np.zeros(self.nelem, dtype=bool)
+            for c in range(8):
+                coarsened[idx + c] = True
+            nb = nbr[idx[:, None] + np.arange(8)[None, :]].astype(np.int64)      # [nf, 8, 6]
+            conf = nb >= 0
+            nbc = np.where(conf, nb, 0)
+            lo = idx[:, None, None]
+            outside = conf & ((nbc < lo) | (nbc >= lo + 8))
+            doomed = (outside & refine[nbc]).any(axis=(1, 2))
+            big = (nb < 0) & ((-nb - 1) % 5 == 0)
+            if len(mortars):
+                m = np.where(big, (-nb - 1) // 5, 0)
+                small = mortars[m][..., 2:6].astype(np.int64)                    # [nf, 8, 6, 4]
+                stay = ~coarsened[small].all(axis=-1)
+                doomed |= (big & stay).any(axis=(1, 2))
+            if not doomed.any():
+                return starts
+            starts[idx[doomed]] = False
+
+    def adapt(self, refine: np.ndarray, coarsen: np.ndarray, prune: bool = True):
         """One adaptation pass (mesh_utils_mod.f90:879-1009).
 
         ``refine[i]``: leaf i wants refinement (honoured below maxlevel).
@@ -294,6 +329,8 @@ class Forest:
             ok &= coarsen[idx + c]
         starts[:] = False
         starts[idx[ok]] = True
+        if prune and starts.any():
+            starts = self._drop_rerefined(starts, refine)
         if not refine.any() and not starts.any():
             return self, None
         new = self.coarsen_refine(starts, refine)

@@ -104,6 +104,55 @@ def test_adapt_units_and_round_trip():
     back.connectivity()  # still 2:1 balanced
 
 
+def test_adapt_pruning_gives_the_same_forest():
+    """Dropping the coarsenings that balance would undo (adapt's pruning)
+    changes neither the new forest nor the replacement units, on random
+    refine/coarsen flags over random balanced forests; a steady no-op pass
+    is recognised as such."""
+    rng = np.random.default_rng(5)
+    checked = changed = 0
+    for trial in range(24):
+        f = amr.Forest.uniform(2 + trial % 2, maxlevel=5 + trial % 2)
+        for lv in range(3):
+            cen = f.lower + 0.5 * f.delta
+            f, _ = f.adapt(np.linalg.norm(cen - rng.random(3), axis=1) < 0.3 - 0.07 * lv + 0.05 * rng.random(),
+                           np.zeros(f.nelem, bool))
+        for k in range(4):
+            refine = rng.random(f.nelem) < rng.random() * 0.3 * (k % 2)
+            coarsen = rng.random(f.nelem) < rng.random()
+            a, ua = f.adapt(refine, coarsen, prune=False)
+            b, ub = f.adapt(refine, coarsen, prune=True)
+            assert (ua is None) == (ub is None)
+            assert np.array_equal(a.level, b.level) and np.array_equal(a.anchor, b.anchor)
+            if ua is not None:
+                assert np.array_equal(ua, ub)
+                changed += 1
+            checked += 1
+            f = a
+    assert checked == 96 and changed > 20
+    # steady state: finest elements flagged at maxlevel, the rest flagged for
+    # coarsening next to finer ones -- nothing changes, with or without pruning
+    f = amr.Forest.uniform(3, minlevel=3, maxlevel=6)
+    for lv in range(3):
+        cen = f.lower + 0.5 * f.delta
+        f, _ = f.adapt(np.linalg.norm(cen - 0.5, axis=1) < 0.2 - 0.03 * lv, np.zeros(f.nelem, bool))
+
+    def flags(g):
+        cen = g.lower + 0.5 * g.delta
+        alpha = np.where((g.level == 6) & (np.linalg.norm(cen - 0.5, axis=1) < 0.1), 0.3, 0.0)
+        from nemo_amd import amr_run
+        return amr_run.adapt_flags(g, alpha)
+
+    for _ in range(4):
+        g, u = f.adapt(*flags(f))
+        if u is None:
+            break
+        f = g
+    r, c = flags(f)
+    assert c.sum() > 0
+    assert f.adapt(r, c, prune=True)[1] is None and f.adapt(r, c, prune=False)[1] is None
+
+
 def test_adapt_refine_coarsen_inverse():
     f = _forest(lmax=5)
     flags = np.zeros(f.nelem, dtype=bool)

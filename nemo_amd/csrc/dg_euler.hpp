@@ -181,13 +181,28 @@ __device__ __forceinline__ void surface_pass(const StageArgs<M, Real>& a, long l
     }
 }
 
+// LSRK update done by the last volume pass itself (x last): the register
+// values of the thread's x-line nodes, the time step, and the LDS array
+// that receives the new state.
+template <int M, int NN>
+struct XUpdate {
+    double dt;
+    double r0[5][M];
+    double (*s_unew)[NN];
+};
+
 // Volume pass of direction D: the thread's M line nodes become node states
 // once; the M(M-1)/2 two-point fluxes (or the weak-form derivative) and the
 // M-1 interior FV interface fluxes read those states.  Only LDS is read.
-template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU>
+// With XUPD (FP64, LSRK, pass x run last) the pass does not store its
+// contributions: it adds them to the accumulated surface and y/z terms and
+// applies the LSRK probe to its own line's nodes (register out to HBM, new
+// state into LDS), with the same arithmetic as the separate update.
+template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU, bool XUPD = false>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
-                                               const double* s_p64, const ElemGeom& geo) {
+                                               const double* s_p64, const ElemGeom& geo,
+                                               const XUpdate<M, BE * M * M * M>* xu = nullptr) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
     constexpr bool BETA = (VOL == VOL_EC);
@@ -297,11 +312,30 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         }
     }
 
+    if constexpr (XUPD) {
+        static_assert(sizeof(Real) == 8, "the fused update is the FP64 path's");
+        // low-storage/4-5/timedisc_probe_mod.f90:31-37, as probe_lsrk
+        const int s = a.stage;
+        const long long EM3 = a.nelem * (long long)M3;
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
+        for (int i = 0; i < M; ++i) {
+            const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
+            const long long gn = e * M3 + nidx<D, M>(i, ta, tb);
 #pragma unroll
-        for (int h = 0; h < 5; ++h) s_acc[h][n] += c[i][h];
+            for (int h = 0; h < 5; ++h) {
+                const double rhs = s_acc[h][n] + c[i][h];
+                const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * xu->r0[h][i] + rhs;
+                if (valid) a.r0[h * EM3 + gn] = reg;
+                xu->s_unew[h][n] = S[i].u[h] + a.rk.lsrk_c[s - 1] * xu->dt * reg;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
+#pragma unroll
+            for (int h = 0; h < 5; ++h) s_acc[h][n] += c[i][h];
+        }
     }
 }
 

@@ -605,16 +605,25 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     // under the 256 of two waves per SIMD there (not for the other volume
     // forms or M = 5).  Unconditional loads -- stage 1 and the SSP scheme
     // read u_in instead and ignore it -- keep the wait counts exact.
+    // With LSRK the x pass then runs last and applies the update to its own
+    // line's nodes (XUPD), so the register is requested in that layout: the
+    // M nodes of the thread's x line (QS = M).
     constexpr bool R0_EARLY = EQ == EQ_EULER && !F32 && VOL == VOL_EC && M <= 4;
+    const bool xlast = R0_EARLY && a.timedisc == TD_LSRK54 && !a.rhs_out;
     double r0e[R0_EARLY ? V : 1][R0_EARLY ? QS : 1];
     if constexpr (R0_EARLY) {
+        static_assert(QS == M, "one x line per thread");
         const double* rsrc = (a.timedisc == TD_LSRK54 && a.stage != 1) ? a.r0 : a.u_in;
+        const int lel = tid / Sh::M2, lline = tid - lel * Sh::M2;
+        const long long ex = e0 + lel < a.nelem ? e0 + lel : 0;
 #pragma unroll
         for (int h = 0; h < V; ++h)
 #pragma unroll
             for (int q = 0; q < QS; ++q) {
                 const int idx = tid + q * NT;
-                r0e[h][q] = rsrc[h * EM3 + (idx < nvalid_nodes ? e0 * M3 + idx : 0)];
+                const long long off = xlast ? ex * M3 + nidx<0, M>(q, lline % M, lline / M)
+                                            : (idx < nvalid_nodes ? e0 * M3 + idx : 0);
+                r0e[h][q] = rsrc[h * EM3 + off];
             }
     }
     __syncthreads();
@@ -639,11 +648,31 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         __syncthreads();
         surface_pass<2, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
         __syncthreads();
-        direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-        __syncthreads();
-        direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-        __syncthreads();
-        direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+        bool fused = false;
+        if constexpr (R0_EARLY) {
+            if (xlast) {
+                direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                XUpdate<M, NN> xu;
+                xu.dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
+#pragma unroll
+                for (int h = 0; h < V; ++h)
+#pragma unroll
+                    for (int q = 0; q < QS; ++q) xu.r0[h][q] = r0e[h][q];
+                xu.s_unew = s_u64;
+                direction_pass<0, M, BE, VOL, SURF, Real, TU, true>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu);
+                fused = true;
+            }
+        }
+        if (!fused) {
+            direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            __syncthreads();
+            direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            __syncthreads();
+            direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+        }
     }
     __syncthreads();
 
@@ -731,6 +760,8 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 const int idx = tid + q * NT;
                 if (idx < nvalid_nodes) s_u64[h][slot<M>(idx)] = st[h][q];
             }
+    } else if (xlast) {
+        // done by the x pass (XUPD)
     } else if (lsrk) {
 #pragma unroll
         for (int h = 0; h < V; ++h) {

@@ -182,13 +182,13 @@ __device__ __forceinline__ void surface_pass(const StageArgs<M, Real>& a, long l
 }
 
 // LSRK update done by the last volume pass itself (x last): the register
-// values of the thread's x-line nodes, the time step, and the LDS array
-// that receives the new state.
-template <int M, int NN>
+// values of the thread's x-line nodes and the time step in, the new state
+// of those nodes out.
+template <int M>
 struct XUpdate {
     double dt;
     double r0[5][M];
-    double (*s_unew)[NN];
+    double unew[5][M];
 };
 
 // Volume pass of direction D: the thread's M line nodes become node states
@@ -197,12 +197,12 @@ struct XUpdate {
 // With XUPD (FP64, LSRK, pass x run last) the pass does not store its
 // contributions: it adds them to the accumulated surface and y/z terms and
 // applies the LSRK probe to its own line's nodes (register out to HBM, new
-// state into LDS), with the same arithmetic as the separate update.
+// state into xu), with the same arithmetic as the separate update.
 template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU, bool XUPD = false>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
                                                const double* s_p64, const ElemGeom& geo,
-                                               const XUpdate<M, BE * M * M * M>* xu = nullptr) {
+                                               XUpdate<M>* xu = nullptr) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
     constexpr bool BETA = (VOL == VOL_EC);
@@ -326,7 +326,7 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
                 const double rhs = s_acc[h][n] + c[i][h];
                 const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * xu->r0[h][i] + rhs;
                 if (valid) a.r0[h * EM3 + gn] = reg;
-                xu->s_unew[h][n] = S[i].u[h] + a.rk.lsrk_c[s - 1] * xu->dt * reg;
+                xu->unew[h][i] = S[i].u[h] + a.rk.lsrk_c[s - 1] * xu->dt * reg;
             }
         }
     } else {

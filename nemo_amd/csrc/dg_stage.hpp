@@ -655,15 +655,47 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 __syncthreads();
                 direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
                 __syncthreads();
-                XUpdate<M, NN> xu;
+                XUpdate<M> xu;
                 xu.dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
 #pragma unroll
                 for (int h = 0; h < V; ++h)
 #pragma unroll
                     for (int q = 0; q < QS; ++q) xu.r0[h][q] = r0e[h][q];
-                xu.s_unew = s_u64;
                 direction_pass<0, M, BE, VOL, SURF, Real, TU, true>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu);
                 fused = true;
+                // The new state of the thread's x line: stored straight to
+                // HBM unless a repair or the CFL partials need the block in
+                // LDS (positivity test as in the repair below).
+                const int lel = tid / Sh::M2, lline = tid - lel * Sh::M2;
+                const int lta = lline % M, ltb = lline / M;
+                const bool lvalid = e0 + lel < a.nelem;
+                bool direct = a.cfl_part == nullptr;
+                if (a.repair) {
+                    bool bad = false;
+#pragma unroll
+                    for (int q = 0; q < QS; ++q) {
+                        double uu[V];
+#pragma unroll
+                        for (int h = 0; h < V; ++h) uu[h] = xu.unew[h][q];
+                        if (!isvalid64<EQ>(a.ph64, uu)) bad = true;
+                    }
+                    if (__any(bad && lvalid)) direct = false;
+                }
+                if (direct) {
+                    if (lvalid) {
+                        const long long eb = (e0 + lel) * M3;
+#pragma unroll
+                        for (int h = 0; h < V; ++h)
+#pragma unroll
+                            for (int q = 0; q < QS; ++q) a.u_out[h * EM3 + eb + nidx<0, M>(q, lta, ltb)] = xu.unew[h][q];
+                    }
+                    return;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int h = 0; h < V; ++h)
+#pragma unroll
+                    for (int q = 0; q < QS; ++q) s_u64[h][bslot<M>(lel * M3, nidx<0, M>(q, lta, ltb))] = xu.unew[h][q];
             }
         }
         if (!fused) {

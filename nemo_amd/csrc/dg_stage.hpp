@@ -263,7 +263,11 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
         if (__any((bad >> k) & 1u)) bad_el |= 1u << k;
 
     // judge / fallback / training data need the PP factor
-    const bool judged = valid && (a.feat_out || sample_hash(e, a.step_index) < a.sample_thresh);
+    // judged blocks are drawn whole (key: the block's first element), so
+    // that the Persson-Peraire factor is computed for the sampled fraction
+    // of the wavefronts rather than for every wavefront holding one sampled
+    // element
+    const bool judged = valid && (a.feat_out || sample_hash(e0, a.step_index) < a.sample_thresh);
     const bool need_pp = __any(judged) || bad_el != 0;
     Real a_pp = Real(0);
     if (need_pp) a_pp = indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64, false);

@@ -194,15 +194,16 @@ struct XUpdate {
 // Volume pass of direction D: the thread's M line nodes become node states
 // once; the M(M-1)/2 two-point fluxes (or the weak-form derivative) and the
 // M-1 interior FV interface fluxes read those states.  Only LDS is read.
-// With XUPD (FP64, LSRK, pass x run last) the pass does not store its
-// contributions: it adds them to the accumulated surface and y/z terms and
-// applies the LSRK probe to its own line's nodes (register out to HBM, new
-// state into xu), with the same arithmetic as the separate update.
+// With XUPD and fuse set (FP64, LSRK, pass x run last) the pass does not
+// store its contributions: it adds them to the accumulated surface and y/z
+// terms and applies the LSRK probe to its own line's nodes (register out to
+// HBM, new state into xu), with the same arithmetic as the separate update.
+// Without fuse it stores them like the other passes.
 template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU, bool XUPD = false>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
                                                const double* s_p64, const ElemGeom& geo,
-                                               XUpdate<M>* xu = nullptr) {
+                                               XUpdate<M>* xu = nullptr, bool fuse = false) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3;
     constexpr bool BETA = (VOL == VOL_EC);
@@ -312,8 +313,10 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         }
     }
 
-    if constexpr (XUPD) {
-        static_assert(sizeof(Real) == 8, "the fused update is the FP64 path's");
+    static_assert(!XUPD || sizeof(Real) == 8, "the fused update is the FP64 path's");
+    bool fused = false;
+    if constexpr (XUPD) fused = fuse;
+    if (fused) {
         // low-storage/4-5/timedisc_probe_mod.f90:31-37, as probe_lsrk
         const int s = a.stage;
         const long long EM3 = a.nelem * (long long)M3;

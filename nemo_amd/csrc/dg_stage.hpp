@@ -652,21 +652,21 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         __syncthreads();
         surface_pass<2, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
         __syncthreads();
-        bool fused = false;
         if constexpr (R0_EARLY) {
+            // x runs last (one code path for both tails: with LSRK it fuses
+            // the update, otherwise it accumulates like the others)
+            direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            __syncthreads();
+            direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            __syncthreads();
+            XUpdate<M> xu;
+            xu.dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
+#pragma unroll
+            for (int h = 0; h < V; ++h)
+#pragma unroll
+                for (int q = 0; q < QS; ++q) xu.r0[h][q] = r0e[h][q];
+            direction_pass<0, M, BE, VOL, SURF, Real, TU, true>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu, xlast);
             if (xlast) {
-                direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-                __syncthreads();
-                direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-                __syncthreads();
-                XUpdate<M> xu;
-                xu.dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
-#pragma unroll
-                for (int h = 0; h < V; ++h)
-#pragma unroll
-                    for (int q = 0; q < QS; ++q) xu.r0[h][q] = r0e[h][q];
-                direction_pass<0, M, BE, VOL, SURF, Real, TU, true>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu);
-                fused = true;
                 // The new state of the thread's x line: stored straight to
                 // HBM unless a repair or the CFL partials need the block in
                 // LDS (positivity test as in the repair below).
@@ -701,8 +701,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
                     for (int q = 0; q < QS; ++q) s_u64[h][bslot<M>(lel * M3, nidx<0, M>(q, lta, ltb))] = xu.unew[h][q];
             }
-        }
-        if (!fused) {
+        } else {
             direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
             __syncthreads();
             direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);

@@ -70,6 +70,10 @@ __device__ __forceinline__ double div_rn(double a, double b) {
     const double q = a * y;
     return fma(fma(-b, q, a), y, q);
 }
+// IEEE-rounded division: div_rn in FP64 (same bits for the operands here),
+// the IEEE division in FP32.
+__device__ __forceinline__ double div_ieee(double a, double b) { return div_rn(a, b); }
+__device__ __forceinline__ float div_ieee(float a, float b) { return a / b; }
 __device__ __forceinline__ double fdiv(double a, double b) { return a * rcp_nr(b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double frcp(double a) { return rcp_nr(a); }
@@ -152,15 +156,7 @@ __device__ __forceinline__ float flog(float a) { return __builtin_amdgcn_logf(a)
 template <typename Real>
 __device__ __forceinline__ Real pressure_ordered(const Gas<Real>& g, Real rho, Real m0, Real m1, Real m2, Real E) {
 #pragma clang fp contract(off)
-    return g.kappa_m1 * (E - Real(0.5) / rho * (m0 * m0 + m1 * m1 + m2 * m2));
-}
-
-// The same pressure for node states of valid (positive-density) data, with
-// the division as div_rn: identical bits.
-__device__ __forceinline__ double pressure_ordered_rn(const Gas<double>& g, double rho, double m0, double m1, double m2,
-                                                      double E) {
-#pragma clang fp contract(off)
-    return g.kappa_m1 * (E - div_rn(0.5, rho) * (m0 * m0 + m1 * m1 + m2 * m2));
+    return g.kappa_m1 * (E - div_ieee(Real(0.5), rho) * (m0 * m0 + m1 * m1 + m2 * m2));
 }
 
 // Unrotated (x-ordered) pressure: isvalid, node primitives, indicator.
@@ -193,7 +189,7 @@ template <int D, typename Real>
 __device__ __forceinline__ Real lambda_max(const Gas<Real>& g, const Real* u) {
     constexpr int MN = Mom<D, 0>::idx;
     const Real p = pressure_dir<D>(g, u);
-    return fabs(u[MN] / u[DENS]) + sqrt(g.kappa * p / u[DENS]);
+    return fabs(div_ieee(u[MN], u[DENS])) + sqrt(div_ieee(g.kappa * p, u[DENS]));
 }
 
 // One node's state as seen from direction D: everything the two-point flux
@@ -221,8 +217,7 @@ __device__ __forceinline__ void make_state(const Gas<Real>& g, NodeState<Real>& 
     s.v[1] = s.u[MOMY] * s.inv_rho;
     s.v[2] = s.u[MOMZ] * s.inv_rho;
     if constexpr (sizeof(Real) == 8) {
-        s.p = pressure_ordered_rn(g, s.u[DENS], s.u[Mom<D, 0>::idx], s.u[Mom<D, 1>::idx], s.u[Mom<D, 2>::idx],
-                                  s.u[ENER]);
+        s.p = pressure_dir<D>(g, s.u);
     } else {
         s.p = g.kappa_m1 * (s.u[ENER] - Real(0.5) * s.inv_rho *
                                             (s.u[MOMX] * s.u[MOMX] + s.u[MOMY] * s.u[MOMY] + s.u[MOMZ] * s.u[MOMZ]));

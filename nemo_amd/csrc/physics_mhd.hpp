@@ -38,8 +38,8 @@ struct Mag {
 template <typename Real>
 __device__ __forceinline__ Real mhd_pressure(const MhdGas<Real>& g, const Real* u) {
 #pragma clang fp contract(off)
-    return g.kappa_m1 * (u[ENER] - Real(0.5) / u[DENS] * (u[MOMX] * u[MOMX] + u[MOMY] * u[MOMY] + u[MOMZ] * u[MOMZ]) -
-                         Real(0.5) / g.mu0 * (u[MAGX] * u[MAGX] + u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ] + u[GLMP] * u[GLMP]));
+    return g.kappa_m1 * (u[ENER] - div_ieee(Real(0.5), u[DENS]) * (u[MOMX] * u[MOMX] + u[MOMY] * u[MOMY] + u[MOMZ] * u[MOMZ]) -
+                         div_ieee(Real(0.5), g.mu0) * (u[MAGX] * u[MAGX] + u[MAGY] * u[MAGY] + u[MAGZ] * u[MAGZ] + u[GLMP] * u[GLMP]));
 }
 
 template <typename Real>

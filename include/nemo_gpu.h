@@ -180,6 +180,13 @@ int nemo_gpu_set_timing(void *handle, int32_t enable);
 /* Measurement aid: one plain device copy of nbytes (reads and writes exactly
  * nbytes), the known-size kernel that scales hardware traffic counters. */
 int nemo_gpu_copy_calibration(void *handle, int64_t nbytes);
+/* Diagnostic, no counterpart in the reference: n generated operand pairs
+ * (random exponents in [-60, 60], half of them nearly equal) through the
+ * kernels' FP64 division (div_rn) and reciprocal (rcp_nr); counts the
+ * results whose bits differ from the IEEE-rounded quotient / reciprocal.
+ * The log-mean's ratio and beta rely on div_rn matching IEEE bit for bit. */
+int nemo_gpu_check_arithmetic(void *handle, int64_t n, uint64_t seed, int64_t *div_mismatches,
+                              int64_t *rcp_mismatches);
 int nemo_gpu_read_timing(void *handle, int64_t *launches, double *total_ms);
 
 /* ---- learned troubled-cell indicator --------------------------------------

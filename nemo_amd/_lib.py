@@ -94,6 +94,8 @@ SIGNATURES = {
     "nemo_gpu_batch_autotune": [_VP, ctypes.c_int32, _PI32, _PI32, _P],
     "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
     "nemo_gpu_copy_calibration": [_VP, ctypes.c_int64],
+    "nemo_gpu_check_arithmetic": [_VP, ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int64)],
     "nemo_gpu_read_timing": [_VP, _PI64, _P],
     "nemo_gpu_operators": [ctypes.c_int32, _P, _P, _P, _P],
     "nemo_gpu_error_message": [ctypes.c_char_p, ctypes.c_int32],

@@ -1044,6 +1044,62 @@ int nemo_gpu_copy_calibration(void* handle, int64_t nbytes) {
     return NEMO_OK;
 }
 
+// Arithmetic self-check: n operand pairs from a counter-based generator
+// (a with a random exponent in [-60, 60] and mantissa; b independent for odd
+// i, a (1 + d) with |d| < 10^-k, k = 1..16, for even i), comparing the bits
+// of div_rn(a, b) with the IEEE quotient and of rcp_nr(b) with the IEEE
+// reciprocal.
+namespace {
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double random_double(unsigned long long h) {
+    const long long ex = 1023 + (long long)((h >> 52) % 121) - 60;
+    return __longlong_as_double((ex << 52) | (long long)(h & 0xFFFFFFFFFFFFFull));
+}
+__global__ void nemo_arith_check_kernel(long long n, unsigned long long seed, unsigned long long* counts) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long h1 = mix64(seed ^ (2ull * i)), h2 = mix64(seed ^ (2ull * i + 1));
+        const double a = random_double(h1);
+        double b;
+        if (i & 1) {
+            b = random_double(h2);
+        } else {
+            double scale = 0.1;
+            for (int k = 0; k < int(h2 & 15); ++k) scale *= 0.1;
+            b = a * (1.0 + (double(h2 >> 11) * 0x1.0p-53 - 0.5) * scale);
+        }
+        if (__double_as_longlong(nemo::div_rn(a, b)) != __double_as_longlong(a / b)) atomicAdd(&counts[0], 1ull);
+        if (__double_as_longlong(nemo::rcp_nr(b)) != __double_as_longlong(1.0 / b)) atomicAdd(&counts[1], 1ull);
+    }
+}
+}  // namespace
+
+int nemo_gpu_check_arithmetic(void* handle, int64_t n, uint64_t seed, int64_t* div_mismatches,
+                              int64_t* rcp_mismatches) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || n <= 0 || !div_mismatches || !rcp_mismatches) return fail(NEMO_ERR_ARG, "bad argument");
+    unsigned long long* d = nullptr;
+    HIP_TRY(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(nemo_arith_check_kernel, dim3(2048), dim3(256), 0, s->stream, (long long)n,
+                           (unsigned long long)seed, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(NEMO_ERR_HIP, std::string("arithmetic check: ") + hipGetErrorString(e));
+    *div_mismatches = int64_t(h[0]);
+    *rcp_mismatches = int64_t(h[1]);
+    return NEMO_OK;
+}
+
 int nemo_gpu_set_timing(void* handle, int32_t enable) {
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail(NEMO_ERR_ARG, "null handle");

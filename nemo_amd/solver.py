@@ -266,6 +266,14 @@ class Solver:
         self._call("nemo_gpu_read_timing", self._h, ctypes.byref(n), ctypes.byref(ms))
         return n.value, ms.value
 
+    def check_arithmetic(self, n: int, seed: int = 1):
+        """(div_rn, rcp_nr) results whose bits differ from IEEE, over n
+        generated operand pairs (nemo_gpu_check_arithmetic)."""
+        d, r = ctypes.c_int64(), ctypes.c_int64()
+        self._call("nemo_gpu_check_arithmetic", self._h, ctypes.c_int64(n), ctypes.c_uint64(seed), ctypes.byref(d),
+                   ctypes.byref(r))
+        return d.value, r.value
+
     # -- driver loop -------------------------------------------------------------
     def mainloop(self, stop_time: float, max_steps: int = 1 << 30, *, dt_checkpoint: float = 0.0,
                  checkpoint_dir: str = None, mesh=None, analysis=None, log=None) -> Runtime:

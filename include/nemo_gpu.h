@@ -3,7 +3,7 @@
  * update for nemo.
  *
  * The reference has no device boundary: its Fortran game loop calls the
- * kernel/equations/timedisc modules directly (SURVEY of the reference:
+ * kernel/equations/timedisc modules directly (call chain in the reference:
  * driver_mod.f90:62-150 -> hook_evolve_500 -> rkstep_cb -> kernel_evolve).
  * This header is the thin layer a Fortran host binds with ISO_C_BINDING
  * (nemo_amd/fortran/nemo_gpu_mod.f90), in the style of the reference's own

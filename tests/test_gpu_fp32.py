@@ -1,8 +1,8 @@
 """FP32 operator (FP64 state, RK accumulation, CFL and repair) against the
 FP64 oracle.  Tolerances: the RHS to FP32 round-off amplified by the
-operator's cancellations (5e-5 relative L-inf), and the north-star accuracy
-bar for the chosen form, L1 density error <= 1e-6 relative to the FP64
-oracle, after 100 steps on the same dt sequence."""
+operator's cancellations (5e-5 relative L-inf), and the accuracy bar of
+the chosen form, L1 density error <= 1e-6 relative to the FP64 oracle,
+after 100 steps on the same dt sequence."""
 import numpy as np
 import pytest
 

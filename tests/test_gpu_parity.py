@@ -86,12 +86,12 @@ def test_timestep_parity():
     assert abs(vm - vmo) <= 1e-15 * vmo
 
 
-@pytest.mark.parametrize("which", ["c1", "c2"])
+@pytest.mark.parametrize("which", ["c1", "c2", "ec-ssp"])
 def test_run_100_steps(which):
-    if which == "c1":
-        N, vol, surf, ind, td, name = COMBOS[0]
-    else:
-        N, vol, surf, ind, td, name = COMBOS[1]
+    """Configs 1 and 2, and the two-point volume form under SSPRK(5,4) (the
+    FP64 kernel's unfused path: with LSRK its last volume pass applies the
+    update itself)."""
+    N, vol, surf, ind, td, name = {"c1": COMBOS[0], "c2": COMBOS[1], "ec-ssp": COMBOS[3]}[which]
     m, u0 = _setup(name, 8, N)
     g, o = _cfgs(N, vol, surf, ind, td, repair=1)
     print(f"100-step parity {which}", flush=True)

@@ -609,9 +609,9 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     // under the 256 of two waves per SIMD there (not for the other volume
     // forms or M = 5).  Unconditional loads -- stage 1 and the SSP scheme
     // read u_in instead and ignore it -- keep the wait counts exact.
-    // With LSRK the x pass then runs last and applies the update to its own
-    // line's nodes (XUPD), so the register is requested in that layout: the
-    // M nodes of the thread's x line (QS = M).
+    // The x pass then runs last and, with LSRK, applies the update to its
+    // own line's nodes (XUPD), so the register is requested in that layout:
+    // the M nodes of the thread's x line (QS = M).
     constexpr bool R0_EARLY = EQ == EQ_EULER && !F32 && VOL == VOL_EC && M <= 4;
     const bool xlast = R0_EARLY && a.timedisc == TD_LSRK54 && !a.rhs_out;
     double r0e[R0_EARLY ? V : 1][R0_EARLY ? QS : 1];
@@ -624,10 +624,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         for (int h = 0; h < V; ++h)
 #pragma unroll
             for (int q = 0; q < QS; ++q) {
-                const int idx = tid + q * NT;
-                const long long off = xlast ? ex * M3 + nidx<0, M>(q, lline % M, lline / M)
-                                            : (idx < nvalid_nodes ? e0 * M3 + idx : 0);
-                r0e[h][q] = rsrc[h * EM3 + off];
+                r0e[h][q] = rsrc[h * EM3 + ex * M3 + nidx<0, M>(q, lline % M, lline / M)];
             }
     }
     __syncthreads();
@@ -804,10 +801,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int idx = tid + q * NT;
-                if constexpr (R0_EARLY)
-                    reg[q] = r0e[h][q];
-                else
-                    reg[q] = (idx < nvalid_nodes && s != 1) ? a.r0[h * EM3 + e0 * M3 + idx] : 0.0;
+                reg[q] = (idx < nvalid_nodes && s != 1) ? a.r0[h * EM3 + e0 * M3 + idx] : 0.0;
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {

@@ -49,26 +49,22 @@ __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, 
         for (int h = 0; h < V; ++h) S[i].u[h] = Real(s_u[h][n]);
         mhd_make_state<BETA>(g, S[i]);
     }
-    Real jac = Real(0);
     Real fm[V], fp[V];
+    // Lanes past the last element read element 0's neighbours (their results
+    // are discarded): unconditional loads, so later loads need not wait.
+    const long long ec = valid ? e : 0;
+    const Real jac = Real(2.0 / a.delta[ec * 3 + D]);
     {
         MhdState<Real> Nm, Np;
-        if (valid) {
-            const long long em = a.nbr[e * 6 + 2 * D];
-            const long long ep = a.nbr[e * 6 + 2 * D + 1];
-            jac = Real(2.0 / a.delta[e * 3 + D]);
+        {
+            const long long em = a.nbr[ec * 6 + 2 * D];
+            const long long ep = a.nbr[ec * 6 + 2 * D + 1];
             const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
             const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
 #pragma unroll
             for (int h = 0; h < V; ++h) {
                 Nm.u[h] = Real(um[h * EM3]);
                 Np.u[h] = Real(up[h * EM3]);
-            }
-        } else {
-#pragma unroll
-            for (int h = 0; h < V; ++h) {
-                Nm.u[h] = S[0].u[h];
-                Np.u[h] = S[M - 1].u[h];
             }
         }
         mhd_make_state<false>(g, Nm);

@@ -24,9 +24,9 @@ __device__ __forceinline__ void mhd_riemann(const MhdGas<Real>& g, const MhdStat
         mhd_rusanov<D>(g, L, R, f);
 }
 
-template <int D, int M, int BE, int VOL, int SURF, typename Real>
+template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU>
 __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, const MhdGas<Real>& g, long long e0,
-                                                   const double (*s_u)[BE * M * M * M],
+                                                   const TU (*s_u)[BE * M * M * M],
                                                    Real (*s_acc)[BE * M * M * M], const Real alpha) {
     using Sh = StageShape<M, BE>;
     constexpr int M2 = Sh::M2, M3 = Sh::M3, V = 9;

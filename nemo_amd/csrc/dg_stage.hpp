@@ -137,7 +137,7 @@ __device__ __forceinline__ Real indicator_pp(const StageArgs<M, Real>& a, long l
     Real* q = s_acc[2];
     for (int idx = tid; idx < NN; idx += NT) {
         const int sl = slot<M>(idx);
-        if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
+        if constexpr (std::is_same<TU, float>::value) {
             q[sl] = Real(double(s_u[DENS][sl]) * s_p64[sl]);
         } else {
             double uu[V];
@@ -225,7 +225,7 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
         for (int idx = tid; idx < NN; idx += NT) {
             const int sl = slot<M>(idx);
             double rho, p;
-            if constexpr (sizeof(Real) == 4 && EQ == EQ_EULER) {
+            if constexpr (std::is_same<TU, float>::value) {
                 rho = double(s_u[DENS][sl]);
                 p = s_p64[sl];
             } else {
@@ -507,15 +507,17 @@ __device__ __forceinline__ void block_cfl(const Phys64& P, const double* __restr
     }
 }
 
-// LDS of one stage block.  FP32 Euler operator: the operator reads an FP32
-// copy of the state (plus the FP64 pressure per node for the HLLC/HLLE
-// blend) and the FP64 new state of the RK update is laid over the
-// operator's LDS afterwards, so a workgroup needs 12 KB instead of 17.5 KB
-// and more of them fit on a CU.
+// LDS of one stage block.  FP32 operator: the operator reads an FP32 copy
+// of the state (plus the FP64 pressure per node, for the indicator and the
+// Euler HLLC/HLLE blend) and the FP64 new state of the RK update is laid
+// over the operator's LDS afterwards, so a workgroup needs 12 KB instead of
+// 17.5 KB (Euler, N=3) or 20 KB instead of 27 KB (GLM-MHD, N=3: eight
+// workgroups per CU, two waves per SIMD, instead of five) and more of them
+// fit on a CU.
 template <int EQ, int M, int BE, typename Real>
 struct StageLds {
     static constexpr int NN = BE * M * M * M, V = NVars<EQ>::V;
-    static constexpr bool COMPACT = sizeof(Real) == 4 && EQ == EQ_EULER;
+    static constexpr bool COMPACT = sizeof(Real) == 4;
     using TU = typename std::conditional<COMPACT, float, double>::type;
     static constexpr int OP_DOUBLES = (V * NN * int(sizeof(TU)) + V * NN * int(sizeof(Real)) + 7) / 8;
     static constexpr int RAW_DOUBLES = OP_DOUBLES > V * NN ? OP_DOUBLES : V * NN;
@@ -598,8 +600,10 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
             for (int h = 0; h < V; ++h) s_u[h][sl] = TU(uu[h][q]);
             if constexpr (COMPACT) {
-                const double un[5] = {uu[0][q], uu[1][q], uu[2][q], uu[3][q], uu[4][q]};
-                s_p64[sl] = pressure(a.ph64.gas, un);
+                double un[V];
+#pragma unroll
+                for (int h = 0; h < V; ++h) un[h] = uu[h][q];
+                s_p64[sl] = pressure64<EQ>(a.ph64, un);
             }
         }
     }
@@ -710,11 +714,12 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 
     // Time derivative of variable h at block node slot sl, in FP64; GLM
     // damping -alpha psi (kernel/mcfv/3d/kernel_flux_mod.f90:496-497) is part
-    // of the FP64 divergence-cleaning update.
-    auto dudt = [&](int h, int sl) {
+    // of the FP64 divergence-cleaning update.  The compact layout holds psi
+    // in FP32 only: its callers pass the node's FP64 value as u64.
+    auto dudt = [&](int h, int sl, double u64) {
         double r = double(s_acc[h][sl]);
         if constexpr (EQ == EQ_MHD)
-            if (h == GLMP) r = r - glm_alpha * double(s_u[GLMP][sl]);
+            if (h == GLMP) r = r - glm_alpha * (COMPACT ? u64 : double(s_u[GLMP][sl]));
         return r;
     };
 
@@ -723,7 +728,8 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
         for (int h = 0; h < V; ++h)
             for (int idx = tid; idx < nvalid_nodes; idx += NT)
-                a.rhs_out[h * EM3 + e0 * M3 + idx] = dudt(h, slot<M>(idx));
+                a.rhs_out[h * EM3 + e0 * M3 + idx] =
+                    dudt(h, slot<M>(idx), COMPACT ? a.u_in[h * EM3 + e0 * M3 + idx] : 0.0);
         return;
     }
 
@@ -779,8 +785,8 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 st[h][q] = 0.0;
                 if (idx < nvalid_nodes) {
                     const long long gi = h * EM3 + e0 * M3 + idx;
-                    st[h][q] = lsrk ? probe_lsrk(a, gi, s, dt, uin[q], reg[q], dudt(h, slot<M>(idx)))
-                                    : probe(gi, uin[q], dudt(h, slot<M>(idx)));
+                    st[h][q] = lsrk ? probe_lsrk(a, gi, s, dt, uin[q], reg[q], dudt(h, slot<M>(idx), uin[q]))
+                                    : probe(gi, uin[q], dudt(h, slot<M>(idx), uin[q]));
                 }
             }
         }
@@ -809,7 +815,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 if (idx < nvalid_nodes) {
                     const long long gi = h * EM3 + e0 * M3 + idx;
                     const int sl = slot<M>(idx);
-                    s_u64[h][sl] = probe_lsrk(a, gi, s, dt, s_u64[h][sl], reg[q], dudt(h, sl));
+                    s_u64[h][sl] = probe_lsrk(a, gi, s, dt, s_u64[h][sl], reg[q], dudt(h, sl, 0.0));
                 }
             }
         }
@@ -819,7 +825,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
             for (int idx = tid; idx < nvalid_nodes; idx += NT) {
                 const long long gi = h * EM3 + e0 * M3 + idx;
                 const int sl = slot<M>(idx);
-                s_u64[h][sl] = probe(gi, s_u64[h][sl], dudt(h, sl));
+                s_u64[h][sl] = probe(gi, s_u64[h][sl], dudt(h, sl, 0.0));
             }
     }
     __syncthreads();

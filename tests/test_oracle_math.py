@@ -363,3 +363,106 @@ def test_isentropic_vortex_convergence():
     eoc = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
     assert errs[-1] < 5e-5
     assert eoc[-1] > 3.0, (errs, eoc)
+
+
+# ---- temporal order of the oracle's stage recursions -------------------------
+
+def _smooth_periodic(m, N):
+    """Smooth, non-uniform periodic state (all five variables vary)."""
+    xi, _ = O.lgl(N)
+    X = m.node_coords(xi)
+    s = np.sin(2 * np.pi * (X[:, 0] + X[:, 1] + X[:, 2]))
+    c = np.cos(2 * np.pi * (X[:, 0] - X[:, 1]))
+    return np.ascontiguousarray(setups.prim2cons(1.0 + 0.2 * s, 0.3 + 0.1 * c, 0.2 - 0.1 * s, 0.1 + 0.05 * c,
+                                                 1.0 + 0.1 * c, GAMMA))
+
+
+def _temporal_eoc(step, cfg, m, u0, T, counts=(2, 4, 8, 16, 32)):
+    """Observed orders log2(d_k / d_{k+1}) of the differences d_k between
+    solutions at T with T/n_k and T/n_{k+1} steps of `step`."""
+    sols = []
+    for n in counts:
+        u = u0.copy()
+        r0, r1 = np.zeros_like(u), np.zeros_like(u)
+        for _ in range(n):
+            if step(cfg, m.nbr, m.delta, u, r0, r1, T / n) != 0:
+                return None
+        sols.append(u)
+    d = [np.abs(sols[i] - sols[i + 1]).max() for i in range(len(sols) - 1)]
+    if not np.all(np.isfinite(d)) or min(d) <= 0:
+        return None
+    return [float(np.log2(d[i] / d[i + 1])) for i in range(len(d) - 1)]
+
+
+_ORDER_CASES = [(O.TD_LSRK54, O.VOL_WEAK, O.SURF_RUSANOV), (O.TD_LSRK54, O.VOL_EC, O.SURF_HLLC),
+                (O.TD_SSPRK54, O.VOL_WEAK, O.SURF_RUSANOV), (O.TD_SSPRK54, O.VOL_EC, O.SURF_HLLC)]
+
+
+@pytest.mark.parametrize("timedisc,volume,surface", _ORDER_CASES)
+def test_oracle_step_temporal_order(timedisc, volume, surface):
+    """O.step itself (not a restatement of its recursion) converges at
+    fourth order in time on a fixed mesh: the semi-discrete ODE is the same
+    for every dt, so successive differences of the solutions at a fixed time
+    shrink by 2^4 per halving of dt (Richardson).  Pins the oracle's LSRK
+    5-4 probe and register recursion (low-storage/4-5/timedisc_probe_mod.f90:
+    31-37, timedisc_hooks_mod.f90:39-47) and the SSPRK(5,4) combinations
+    (ssp/4-5/timedisc_probe_mod.f90:24-46, timedisc_hooks_mod.f90:40-67)."""
+    N = 3
+    m = meshmod.uniform_periodic(4, 4, 4)
+    u0 = _smooth_periodic(m, N)
+    cfg = _cfg(N=N, volume=volume, surface=surface, timedisc=timedisc, cfl=0.5)
+    dt0, _, _ = O.calc_timestep(cfg, m.delta, u0)
+    eoc = _temporal_eoc(O.step, cfg, m, u0, 8 * dt0)
+    assert eoc is not None and min(eoc[1:]) >= 3.8, eoc
+
+
+# Mistakes planted in a private copy of the oracle's stage recursion; each
+# must make the order test above fail.
+_PLANTED = [
+    ("lsrk: a_s mis-indexed", O.TD_LSRK54,
+     "la[s - 1] * q0[n] + fe[n];", "la[s % 5] * q0[n] + fe[n];"),
+    ("lsrk: c_s mis-indexed", O.TD_LSRK54,
+     "st[n] = ue[n] + lc[s - 1] * dt * reg;", "st[n] = ue[n] + lc[s % 5] * dt * reg;"),
+    ("lsrk: old register used", O.TD_LSRK54,
+     "st[n] = ue[n] + lc[s - 1] * dt * reg;", "st[n] = ue[n] + lc[s - 1] * dt * q0[n];"),
+    ("lsrk: a and c swapped", O.TD_LSRK54,
+     "la[s - 1] * q0[n] + fe[n];", "lc[s - 1] * q0[n] + fe[n];"),
+    ("ssp: B1/B2 swapped", O.TD_SSPRK54,
+     "case 2: st[n] = B1 * q0[n] + B2 * ue[n]", "case 2: st[n] = B2 * q0[n] + B1 * ue[n]"),
+    ("ssp: E5 register term dropped", O.TD_SSPRK54,
+     "case 3: q1[n] = q1[n] + E5 * st[n];", "case 3: q1[n] = q1[n];"),
+]
+
+
+@pytest.mark.parametrize("name,timedisc,old,new", _PLANTED, ids=[p[0] for p in _PLANTED])
+def test_planted_rk_mistakes_fail_the_order_test(tmp_path, name, timedisc, old, new):
+    import ctypes
+    import os
+    import shutil
+    import subprocess
+    src = open(os.path.join(os.path.dirname(O.__file__), "nemo_oracle.c")).read()
+    assert src.count(old) == 1, name
+    d = tmp_path / "mut"
+    d.mkdir()
+    (d / "nemo_oracle.c").write_text(src.replace(old, new))
+    shutil.copy(os.path.join(os.path.dirname(O.__file__), "nemo_oracle.h"), d / "nemo_oracle.h")
+    so = d / "libmut.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-fopenmp", "-std=c11", "-shared", "-o", str(so),
+                    str(d / "nemo_oracle.c"), "-lm"], check=True)
+    L = ctypes.CDLL(str(so))
+    P, PI = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+    L.orc_step.argtypes = [ctypes.POINTER(O.Cfg), ctypes.c_int64, PI, P, P, P, P, ctypes.c_double]
+
+    def mut_step(cfg, nbr, delta, u, r0, r1, dt):
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        de = np.ascontiguousarray(delta, dtype=np.float64)
+        return L.orc_step(ctypes.byref(cfg), u.shape[0], nb.ctypes.data_as(PI), de.ctypes.data_as(P),
+                          u.ctypes.data_as(P), r0.ctypes.data_as(P), r1.ctypes.data_as(P), dt)
+
+    N = 3
+    m = meshmod.uniform_periodic(4, 4, 4)
+    u0 = _smooth_periodic(m, N)
+    cfg = _cfg(N=N, volume=O.VOL_WEAK, surface=O.SURF_RUSANOV, timedisc=timedisc, cfl=0.5)
+    dt0, _, _ = O.calc_timestep(cfg, m.delta, u0)
+    eoc = _temporal_eoc(mut_step, cfg, m, u0, 8 * dt0)
+    assert eoc is None or min(eoc[1:]) < 3.8, (name, eoc)

@@ -266,26 +266,68 @@ def amr_sedov(steps: int, seed: int, lmin: int = 3, lmax: int = 6, every: int = 
     return res
 
 
-def cpu_baseline(n: int, N: int, seed: int, steps: int, cfg_kw):
-    """FP64 oracle on the host cores: one or more full steps of the same
-    64^3 workload, after one untimed warm-up stage sequence on a small mesh."""
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n: int, N: int, seed: int, steps: int, cfg_kw, u_gpu=None):
+    """FP64 oracle on the host cores, timed as the reference times itself
+    (its performance index, driver_mod.f90:83,142; OpenMP over the cores
+    with OMP_PROC_BIND=close and OMP_PLACES=cores as in its run script):
+    one untimed warm-up step on an 8^3 mesh, then `steps` full steps of the
+    same n^3 workload on every core this process may use (its CPU affinity,
+    or OMP_NUM_THREADS where the environment sets it lower), then one step
+    of a 32^3 sample on one core.  With u_gpu (the GPU state after the same
+    steps from the same initial data) it also reports the relative Linf
+    difference at full size."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(cores, int(env)) if env and env.isdigit() and int(env) > 0 else cores
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
     kw = dict(cfg_kw)
     ocfg = O.make_cfg(N=N, volume=kw["volume"], surface=kw["surface"], indicator=kw["indicator"],
                       timedisc=kw["timedisc"], repair=kw["repair"], cfl=kw["cfl"])
+    O.set_num_threads(threads)
+    mw, uw = workload(8, N, seed)
+    O.run(ocfg, mw.nbr, mw.delta, uw, 1)  # warm-up
     m, u = workload(n, N, seed)
     t0 = time.perf_counter()
     rc, _ = O.run(ocfg, m.nbr, m.delta, u, steps)
     wall = time.perf_counter() - t0
     dof = m.nelem * (N + 1) ** 3
-    return {"value": dof * NSTAGES * steps / wall, "unit": "DOF-updates/s", "cores": O.num_threads(),
-            "kind": "port",
-            "sample": f"{n}^3 elements, N={N}, {steps} full LSRK 5-4 step(s) of the same KH workload, "
-                      f"{wall:.1f} s wall, FP64 C oracle, OpenMP"}
+    value = dof * NSTAGES * steps / wall
+    used = O.num_threads()
+    # one core, smaller sample of the same workload
+    O.set_num_threads(1)
+    m1, u1 = workload(32, N, seed)
+    t0 = time.perf_counter()
+    O.run(ocfg, m1.nbr, m1.delta, u1, 1)
+    wall1 = time.perf_counter() - t0
+    v1 = m1.nelem * (N + 1) ** 3 * NSTAGES / wall1
+    O.set_num_threads(threads)
+    out = {"value": value, "unit": "DOF-updates/s", "cores": used, "kind": "port",
+           "sample": f"{n}^3 elements, N={N}, {steps} full LSRK 5-4 step(s) of the same KH workload after a "
+                     f"one-step warm-up on 8^3, {wall:.1f} s wall, FP64 C oracle, OpenMP on {used} threads",
+           "cpu_model": _cpu_model(), "cores_available": cores,
+           "perf_index_us_per_dof_stage_core": 1e6 * used / value,
+           "one_core": {"value": v1, "unit": "DOF-updates/s", "sample": f"32^3 elements, 1 step, {wall1:.1f} s",
+                        "perf_index_us_per_dof_stage_core": 1e6 / v1},
+           "status_ok": rc == 0}
+    if u_gpu is not None:
+        out["parity_full_size"] = {"steps": steps,
+                                   "rel_linf_gpu_vs_oracle": float(np.abs(u_gpu - u).max() / np.abs(u).max()),
+                                   "bar": 1e-10}
+    return out
 
 
 def main():
@@ -399,7 +441,16 @@ def main():
     if not args.no_amr:
         out["amr_sedov"] = amr_sedov(args.amr_steps, args.seed)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.elems, N, args.seed, args.cpu_steps, cfg_kw)
+        # the GPU state after the same steps from the same data: a parity
+        # point at full size for the CPU leg to compare with
+        from nemo_amd import solver
+        with solver.Solver(solver.make_config(N=N, precision=0, **cfg_kw)) as s:
+            s.upload_mesh(m)
+            s.set_state(u0)
+            s.run(args.cpu_steps)
+            u_gpu = s.get_state()
+        out["cpu_baseline"] = cpu_baseline(args.elems, N, args.seed, args.cpu_steps, cfg_kw, u_gpu)
+        del u_gpu
     print(json.dumps(out), flush=True)
     if args.calibrate:
         from nemo_amd import solver

@@ -38,6 +38,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* Thread count of the following parallel regions (CPU baseline timing). */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int orc_nvars(const orc_cfg *c) { return c->eq == ORC_EQ_MHD ? 9 : 5; }
 
 /* ======================================================================== */

@@ -131,6 +131,7 @@ void orc_lsrk54_coeffs(double *a, double *b, double *cc);
 void orc_ssprk54_coeffs(double *coef /* 23 values A0..E6 */);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -400,6 +400,11 @@ def num_threads() -> int:
     return lib().orc_num_threads()
 
 
+def set_num_threads(n: int) -> None:
+    """OpenMP thread count of the oracle's following parallel regions."""
+    lib().orc_set_num_threads(int(n))
+
+
 # ---- the reference's own Fortran leaf modules (oracle/_ref) ---------------
 
 class Ref:

@@ -125,7 +125,11 @@ int nemo_gpu_download_state(void *handle, double *u);
  * hook_pre_rk_timestep_300).  Parity entry point. */
 int nemo_gpu_rhs(void *handle, double *rhs);
 
-/* Blending factor per element for the current state (indicator only). */
+/* Blending factor per element for the current state (indicator only).
+ * nemo_gpu_rhs, nemo_gpu_indicator and nemo_gpu_indicator_features are
+ * diagnostics: with the learned indicator they evaluate a fresh factor but
+ * leave the factor of the current time step (used by stages 2..5) and the
+ * judge's counters untouched, so they may be called between stages. */
 int nemo_gpu_indicator(void *handle, double *alpha);
 
 /* Volume integrals of the current state with the LGL quadrature volumes

@@ -277,7 +277,6 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
 #pragma unroll
             for (int k = 0; k < 4; ++k) a.feat_out[e * 4 + k] = x[k];
             if (a.alpha_out) a.alpha_out[e] = double(a_pp);
-            a.alpha_step[e] = double(a_pp);
         }
         return a_pp;
     }
@@ -310,10 +309,13 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
         alpha = double(a_pp);
         code |= 4;
     }
+    // A diagnostic launch (RHS, indicator or feature output, rhs_out set)
+    // reports the factor through alpha_out only: the step's factor and the
+    // judge's counters belong to the stage sequence.
     if ((tid & 15) == 0 && valid) {
-        a.alpha_step[e] = alpha;
+        if (!a.rhs_out) a.alpha_step[e] = alpha;
         if (a.alpha_out) a.alpha_out[e] = alpha;
-        if (a.mlp_stats && code) {
+        if (a.mlp_stats && code && !a.rhs_out) {
             if (code & 1) atomicAdd(&a.mlp_stats[0], 1ull);
             if (code & 2) atomicAdd(&a.mlp_stats[1], 1ull);
             if (code & 4) atomicAdd(&a.mlp_stats[2], 1ull);

@@ -244,6 +244,10 @@ class Solver:
             self._call("nemo_gpu_stage", self._h, ctypes.c_int32(stage), ctypes.c_double(rt.timestep))
         self._call("nemo_gpu_synchronize", self._h)
 
+    def stage(self, stage: int, dt: float) -> None:
+        """One RK stage (1..5) with time step dt, asynchronous."""
+        self._call("nemo_gpu_stage", self._h, ctypes.c_int32(stage), ctypes.c_double(dt))
+
     def step(self, dt: float) -> None:
         self._call("nemo_gpu_step", self._h, ctypes.c_double(dt))
 

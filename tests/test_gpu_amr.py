@@ -208,3 +208,134 @@ def test_adaptive_sedov_run_conserves_and_replays():
         amr_run.replay(s2, run.history)
         u2 = s2.get_state()
     assert np.array_equal(u1, u2)
+
+
+def _oracle_replay(history, u0, N, base_cfg, dt_check=True):
+    """The recorded meshes, state transfers and time steps of an adaptive
+    run, replayed through the oracle (O.remap, O.step with the mesh's
+    mortars); returns the final state, the fallback count and the largest
+    relative difference between a recorded dt and the oracle's own CFL dt."""
+    u = u0.copy()
+    fallbacks, ddt = 0, 0.0
+    for seg in history:
+        if seg.units is not None:
+            u, nfb = O.remap(N, 1.4, u, seg.units, seg.forest.nelem)
+            fallbacks += nfb
+        o = O.with_mortars(base_cfg, seg.forest.mortars)
+        r0, r1 = np.zeros_like(u), np.zeros_like(u)
+        for dt in seg.dts:
+            if dt_check:
+                dto, _, _ = O.calc_timestep(o, seg.forest.delta, u)
+                ddt = max(ddt, abs(float(dt) - dto) / dto)
+            assert O.step(o, seg.forest.nbr, seg.forest.delta, u, r0, r1, float(dt)) == 0
+    return u, fallbacks, ddt
+
+
+def test_adaptive_sedov_against_oracle():
+    """Adaptive Sedov blast (levels 2-4, indicator-driven adaptation every 8
+    steps, 40 steps, FP64 + Persson-Peraire) against the oracle replaying the
+    same meshes: every state transfer through O.remap, every step through
+    O.step with the mesh's mortars.  State <= 1e-10 relative, each recorded
+    dt equal to the oracle's CFL dt to 1e-12, equal fallback counts.  The
+    chosen form (FP32 operator + learned indicator, judge on) and the FP32
+    operator alone replay the same meshes and time steps; their L1 density
+    departures from the oracle are bounded (the learned indicator's by the
+    judge tolerance's effect, the FP32 operator's by the 1e-6 bar)."""
+    from nemo_amd import amr_run, indicator_mlp
+    N = 3
+    kw = dict(N=N, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC, timedisc=_lib.TD_LSRK54, repair=1)
+    cfg = solver.make_config(indicator=_lib.IND_PP, **kw)
+    ic = lambda f: indicator_mlp.sedov(f, N, radius=0.04, p0=1e-3, seed=5)
+    f0 = amr_run.initial_forest(lambda: solver.Solver(cfg), amr.Forest.uniform(2, minlevel=2, maxlevel=4), ic)
+    u0 = ic(f0)
+    print(f"adaptive sedov vs oracle: {f0.nelem} initial elements", flush=True)
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(f0)
+        s.set_state(u0)
+        run = amr_run.AdaptiveRun(s, f0, every=8)
+        run.advance(40, record=True)
+        ug = s.get_state()
+    assert run.remeshes >= 2
+    uo, nfb, ddt = _oracle_replay(run.history, u0, N, O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC,
+                                                                 indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+                                                                 repair=1))
+    err = _rel(ug, uo)
+    print(f"  elements per segment {run.elements}; state {err:.2e}, dt {ddt:.2e}, fallbacks {nfb}/{run.fallbacks}")
+    assert nfb == run.fallbacks
+    assert ddt <= 1e-12
+    assert err <= RUN_TOL
+
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    vol = np.prod(run.forest.delta, 1)[:, None] * w3[None, :] / 8.0
+    l1 = lambda u: float(np.sum(np.abs(u[:, 0] - uo[:, 0]) * vol) / np.sum(np.abs(uo[:, 0]) * vol))
+    legs = {"fp32_pp": solver.make_config(indicator=_lib.IND_PP, precision=1, **kw),
+            "chosen": solver.make_config(indicator=_lib.IND_MLP, precision=1, **kw)}
+    errs = {}
+    for name, c in legs.items():
+        with solver.Solver(c) as s:
+            s.upload_mesh(f0)
+            s.set_state(u0)
+            if c.indicator == _lib.IND_MLP:
+                s.mlp_set_params(indicator_mlp.load_params())
+            amr_run.replay(s, run.history)
+            errs[name] = l1(s.get_state())
+            stats = s.mlp_stats() if c.indicator == _lib.IND_MLP else None
+        print(f"  {name}: L1 density vs oracle {errs[name]:.2e}" + (f", judge {stats}" if stats else ""))
+    assert errs["fp32_pp"] <= 1e-6
+    assert errs["chosen"] <= 1e-4
+
+
+def test_sedov_levels_3_to_6_chosen_form():
+    """The adaptive Sedov configuration itself (E = 1 in a Gaussian of radius
+    0.02, p0 = 1e-5, levels 3-6, adaptation every 8 steps) with the chosen
+    form: FP32 operator, learned indicator judged on a sampled 5% per step.
+    48 steps: mass and energy are conserved through the adaptations (FP32
+    operator, FP64 update), the run replays bit for bit on a second solver,
+    the judge sampled about 5% of the element-steps and the network kept
+    the blast positive without repair failures."""
+    from nemo_amd import amr_run, indicator_mlp
+    N = 3
+    kw = dict(N=N, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC, timedisc=_lib.TD_LSRK54, repair=1)
+    cfg64 = solver.make_config(indicator=_lib.IND_PP, **kw)
+    cfg = solver.make_config(indicator=_lib.IND_MLP, precision=1, **kw)
+    params = indicator_mlp.load_params()
+    ic = lambda f: indicator_mlp.sedov(f, N, seed=1234, radius=0.02, p0=1e-5)
+    f0 = amr_run.initial_forest(lambda: solver.Solver(cfg64), amr.Forest.uniform(3, minlevel=3, maxlevel=6), ic)
+    assert f0.level.max() == 6 and f0.level.min() == 3
+    u0 = ic(f0)
+    print(f"sedov levels 3-6: {f0.nelem} initial elements", flush=True)
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    mass = lambda f, u: np.einsum("evn,n,e->v", u, w3, np.prod(f.delta, 1) / 8.0)
+    steps = 48
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(f0)
+        s.set_state(u0)
+        s.mlp_set_params(params)
+        s.mlp_stats()
+        run = amr_run.AdaptiveRun(s, f0, every=8)
+        run.advance(steps, record=True)
+        u1 = s.get_state()
+        judged, overridden, fallback = s.mlp_stats()
+        f1 = run.forest
+    ext = sum(len(seg.dts) * seg.forest.nelem for seg in run.history)
+    print(f"  elements per segment {run.elements}, remeshes {run.remeshes}; judge {judged}/{ext} "
+          f"overridden {overridden} fallback {fallback}")
+    assert run.remeshes >= 1
+    m0, m1 = mass(f0, u0), mass(f1, u1)
+    dm = np.abs(m1[[0, 4]] - m0[[0, 4]]) / np.abs(m0[[0, 4]])
+    print(f"  relative mass / energy change {dm[0]:.2e} / {dm[1]:.2e}")
+    assert dm.max() <= 1e-10
+    assert 0.03 <= judged / ext <= 0.07
+    assert overridden <= judged
+    assert u1[:, 0].min() > 0.0
+    with solver.Solver(cfg) as s2:
+        s2.upload_mesh(f0)
+        s2.set_state(u0)
+        s2.mlp_set_params(params)
+        s2.mlp_stats()
+        amr_run.replay(s2, run.history)
+        u2 = s2.get_state()
+        assert s2.mlp_stats() == (judged, overridden, fallback)
+    assert np.array_equal(u1, u2)

@@ -82,3 +82,78 @@ def test_autotune_restores_state():
         ub = [b.download(r) for r in range(len(states))]
     for a, c in zip(ua, ub):
         assert np.array_equal(a, c)
+
+
+def _oracle_cfg(N):
+    from oracle import oracle as O
+    return O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+                      repair=1, cfl=0.5, alpha_max=0.5, alpha_min=0.001)
+
+
+def _uneven_riemann_runs(seed):
+    """Five planar Riemann problems of uneven size and degree (3^3..6^3
+    elements, N in {2, 3, 4}): the ensemble workload at test scale."""
+    rng = np.random.default_rng(seed)
+    spec = [(3, 2), (5, 3), (4, 4), (6, 2), (4, 3)]
+    meshes, npolys, states = [], [], []
+    for n, N in spec:
+        m = meshmod.uniform_periodic(n)
+        meshes.append(m)
+        npolys.append(N)
+        states.append(setups.planar_riemann(m, N, rng))
+    return meshes, npolys, states
+
+
+def test_batch_fp64_against_oracle():
+    """The batched path itself (ghost-padded blocks, per-run time steps from
+    the per-run CFL reduction, graph replay of the stage sequence) against
+    the FP64 oracle run by run: 20 LSRK 5-4 steps with blending and repair,
+    state <= 1e-10 and dt history <= 1e-12 relative."""
+    from oracle import oracle as O
+    meshes, npolys, states = _uneven_riemann_runs(seed=21)
+    steps = 20
+    cfg = solver.make_config(N=3, precision=0, **KW)
+    print("batch fp64 vs oracle", flush=True)
+    with solver.Batch(cfg, meshes, npolys) as b:
+        for r, u in enumerate(states):
+            b.upload(r, u)
+        dts = b.run(steps, use_graph=True, record_dt=True)
+        ub = [b.download(r) for r in range(len(states))]
+    blended = 0
+    for r, (m, N, u0) in enumerate(zip(meshes, npolys, states)):
+        uo = u0.copy()
+        rc, dto = O.run(_oracle_cfg(N), m.nbr, m.delta, uo, steps)
+        assert rc == 0
+        ddt = np.abs(dts[:, r] - dto).max() / dto.max()
+        err = np.abs(ub[r] - uo).max() / np.abs(uo).max()
+        a, _ = O.indicator(_oracle_cfg(N), uo)
+        blended += int((a > 0).sum())
+        print(f"  run {r}: {m.nelem} el N={N}: state {err:.2e} dt {ddt:.2e}")
+        assert ddt <= 1e-12, r
+        assert err <= 1e-10, r
+    assert blended > 0  # the FV subcells took part
+
+
+def test_batch_fp32_against_oracle():
+    """FP32 operator in the batch (FP64 state, RK, CFL and repair): L1
+    density error <= 1e-6 relative to the FP64 oracle after 20 steps, run
+    by run (each on its own CFL time steps)."""
+    from oracle import oracle as O
+    meshes, npolys, states = _uneven_riemann_runs(seed=22)
+    steps = 20
+    cfg = solver.make_config(N=3, precision=1, **KW)
+    print("batch fp32 vs oracle", flush=True)
+    with solver.Batch(cfg, meshes, npolys) as b:
+        for r, u in enumerate(states):
+            b.upload(r, u)
+        b.run(steps, use_graph=True)
+        ub = [b.download(r) for r in range(len(states))]
+    for r, (m, N, u0) in enumerate(zip(meshes, npolys, states)):
+        uo = u0.copy()
+        rc, _ = O.run(_oracle_cfg(N), m.nbr, m.delta, uo, steps)
+        assert rc == 0
+        _, w = O.lgl(N)
+        W = np.einsum("i,j,k->kji", w, w, w).reshape(-1)[None, :]
+        l1 = float((W * np.abs(ub[r][:, 0] - uo[:, 0])).sum() / (W * np.abs(uo[:, 0])).sum())
+        print(f"  run {r}: {m.nelem} el N={N}: L1 density {l1:.2e}")
+        assert l1 <= 1e-6, r

@@ -74,6 +74,38 @@ def test_mhd_rhs_parity(combo):
     assert max(err) <= RHS_TOL
 
 
+@pytest.mark.parametrize("ind", [_lib.IND_NONE, _lib.IND_PP])
+def test_mhd_fp32_rhs_against_fp64_oracle(ind):
+    """FP32 operator's RHS on the rough state (density and pressure jumps,
+    non-zero psi, GLM speed and damping set) against the FP64 oracle, per
+    variable: the compact FP32 layout's FP64 psi for the damping term and
+    its indicator on FP32-rounded rho are both on this path.  Bound as for
+    the Euler FP32 RHS: 5e-5 of the variable's max."""
+    m = meshmod.uniform_periodic(6)
+    u0 = _state(m, rough=True)
+    glm = (0.8, 4.0)
+    g, o = _cfgs(_lib.VOL_EC, _lib.SURF_HLLD, ind, precision=1, glm=glm)
+    print(f"mhd fp32 rhs ind={ind}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        s.set_glm(*glm)
+        r = s.kernel_calc_flux()
+        a = s.indicator()
+    aref = O.indicator(o, u0)[0] if ind == _lib.IND_PP else None
+    if ind == _lib.IND_PP:
+        assert (aref > 0).any()
+        assert np.abs(a - aref).max() <= 1e-4
+    rref = O.rhs(o, m.nbr, m.delta, u0, aref)
+    err = _rel_per_var(r, rref)
+    print("  rel Linf per var", " ".join(f"{e:.2e}" for e in err))
+    assert max(err) <= 5e-5
+    # the damping term -alpha psi is in the psi equation: dropping it moves
+    # that variable far beyond the bound
+    damp = 4.0 * np.abs(u0[:, 8]).max() / np.abs(rref[:, 8]).max()
+    assert damp > 1e-2
+
+
 def test_mhd_timestep_and_glm():
     m = meshmod.uniform_periodic(6)
     u0 = _state(m, rough=True)

@@ -210,3 +210,35 @@ def test_trained_network_gain_on_sedov():
                   f" pp {rmin['pp']:.3e} mlp {rmin['mlp']:.3e}")
             assert d < 0.05 * dn
             assert rmin["mlp"] > 5.0 * rn
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_diagnostics_between_stages_leave_the_step_untouched(precision):
+    """nemo_gpu_rhs, nemo_gpu_indicator and nemo_gpu_indicator_features called
+    between the stages of a step evaluate a fresh factor, but neither the
+    factor that stages 2..5 use nor the judge's counters change: the step
+    ends bit for bit where an uninterrupted step ends."""
+    m = meshmod.uniform_periodic(8)
+    u0 = indicator_mlp.sedov(m, 3, seed=7, radius=0.05)
+    params = indicator_mlp.init_params(5)
+    out = {}
+    for interrupt in (False, True):
+        with solver.Solver(_cfg(ind=_lib.IND_MLP, precision=precision)) as s:
+            s.upload_mesh(m)
+            s.set_state(u0)
+            s.mlp_set_params(params)
+            s.mlp_set_policy(1.0, 0.0)   # every element judged: the counters move each step
+            dt, _, _ = s.hook_calc_timestep()
+            s.stage(1, dt)
+            if interrupt:
+                s.kernel_calc_flux()
+                a_mid = s.indicator()
+                s.indicator_features()
+            for st in range(2, 6):
+                s.stage(st, dt)
+            s.synchronize()
+            out[interrupt] = (s.get_state(), s.mlp_stats())
+    assert np.array_equal(out[False][0], out[True][0])
+    assert out[False][1] == out[True][1] and out[False][1][0] == m.nelem
+    # blending is active, so a factor written back would change stages 2..5
+    assert np.any(a_mid > 0)

@@ -133,6 +133,10 @@ def test_judge_and_fallback():
         s.mlp_set_params(params)
         s.mlp_set_policy(1.0, 0.0)   # judge every element, no tolerance
         a = s.indicator()
+        # the judge's counters belong to the stage sequence: a first stage
+        # (dt = 0 leaves the state as it is) refreshes and judges the factor
+        s.stage(1, 0.0)
+        s.synchronize()
         judged, overridden, fallback = s.mlp_stats()
         f, a_pp = s.indicator_features()
     y, _ = mlp_ref.forward(params.astype(np.float64), f.astype(np.float64))

@@ -141,6 +141,22 @@ int nemo_gpu_indicator(void *handle, double *alpha);
 #define NEMO_ANALYZE_NSUMS 16
 int nemo_gpu_analyze(void *handle, double *sums);
 
+/* Allocation-free nemo_gpu_analyze.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Enqueues the same kernels on the handle's
+ * stream and returns without synchronising; sums_dev is a device buffer of
+ * NEMO_ANALYZE_NSUMS doubles owned by the caller, filled when the stream
+ * reaches the call (nemo_gpu_synchronize waits for it).  The workgroup
+ * partials live in a fixed array of the library (8 MiB per GPU, held until
+ * the process ends) instead of memory allocated on first use, so the call
+ * allocates nothing; meshes of more than NEMO_ANALYZE_NOALLOC_MAX_NODES nodes
+ * per variable (64^3 elements at n_poly = 3) are rejected with
+ * NEMO_ERR_ARG.  All calls on one GPU share that array: calls whose work
+ * overlaps on one GPU (several handles, several streams or host threads)
+ * race and must be ordered by the caller. */
+#define NEMO_ANALYZE_NOALLOC_MAX_NODES 16777216
+int nemo_gpu_analyze_noalloc(void *handle, double *sums_dev);
+
 /* CFL time step of the current state (replaces hook_calc_timestep_200,
  * kernel/dgsem/weakform/2d/kernel_timestep_mod.f90:19-63): dt =
  * 0.5*CFL/(2*N_NODES-1)*min(delta/lambda); lmax and vmax are the global

@@ -68,6 +68,7 @@ SIGNATURES = {
     "nemo_gpu_rhs": [_VP, _P],
     "nemo_gpu_indicator": [_VP, _P],
     "nemo_gpu_analyze": [_VP, _P],
+    "nemo_gpu_analyze_noalloc": [_VP, _VP],
     "nemo_gpu_calc_timestep": [_VP, _P, _P, _P],
     "nemo_gpu_set_glm": [_VP, ctypes.c_double, ctypes.c_double],
     "nemo_gpu_get_glm": [_VP, _P, _P],

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -18,9 +19,39 @@
 #include "dg_mortar.hpp"
 #include "dg_analyze.hpp"
 
+// Workgroup partials of nemo_gpu_analyze_noalloc: one row of
+// NEMO_ANALYZE_NSUMS doubles per analysis workgroup of 256 nodes, enough for
+// NEMO_ANALYZE_NOALLOC_MAX_NODES nodes per variable (8 MiB).  Every GPU the
+// library runs on holds its own copy until the process ends; all calls on
+// one GPU share it.  Host code never passes this name to a kernel: its device
+// address comes from hipGetSymbolAddress (scratch_device_ptr below).
+static __device__ __align__(256) double scratch[NEMO_ANALYZE_NOALLOC_MAX_NODES / nemo::AN_NT * nemo::AN_NSUMS];
+
 namespace {
 
 using namespace nemo_host;
+
+// Device address of `scratch` on the current device, resolved once per
+// device (with that device current) and kept for later calls.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+hipError_t scratch_device_ptr(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_dev[dev];
+    return hipSuccess;
+}
 
 // ---------------------------------------------------------------------------
 // Solver
@@ -49,6 +80,7 @@ struct Solver {
     double* an_part = nullptr;         // analysis partials
     long long an_cap = 0;
     double* an_out = nullptr;          // [NEMO_ANALYZE_NSUMS]
+    double* scratch = nullptr;         // this device's `scratch` (allocation-free analysis)
     double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
     double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;
@@ -428,11 +460,11 @@ int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, 
 }
 
 template <int M>
-void launch_analyze(Solver& s, long long nblocks, bool mhd) {
+void launch_analyze(Solver& s, long long nblocks, bool mhd, double* part) {
     nemo::AnalyzeArgs<M> a{};
     a.u = s.u[s.cur];
     a.delta = s.delta;
-    a.part = s.an_part;
+    a.part = part;
     a.nelem = s.nelem;
     a.ph = make_phys64(s.cfg);
     for (int i = 0; i < M; ++i) a.w[i] = s.ops.w[i];
@@ -536,6 +568,12 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         (void)hipStreamDestroy(s->stream);
         delete s;
         return fail(NEMO_ERR_HIP, "hipMalloc failed");
+    }
+    // this device's copy of the allocation-free analysis scratch
+    if (hipError_t e = scratch_device_ptr(&s->scratch); e != hipSuccess) {
+        s->scratch = nullptr;
+        nemo_gpu_destroy(s);
+        return fail(NEMO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     }
     (void)hipMemset(s->status, 0, sizeof(int));
     (void)hipMemset(s->dt_dev, 0, 8 * sizeof(double));
@@ -701,9 +739,9 @@ int nemo_gpu_analyze(void* handle, double* sums) {
     if (!s->an_out) HIP_TRY(hipMalloc(&s->an_out, nemo::AN_NSUMS * sizeof(double)));
     const bool mhd = s->cfg.equations == NEMO_EQ_GLM_MHD;
     switch (s->M) {
-    case 3: launch_analyze<3>(*s, nblocks, mhd); break;
-    case 4: launch_analyze<4>(*s, nblocks, mhd); break;
-    case 5: launch_analyze<5>(*s, nblocks, mhd); break;
+    case 3: launch_analyze<3>(*s, nblocks, mhd, s->an_part); break;
+    case 4: launch_analyze<4>(*s, nblocks, mhd, s->an_part); break;
+    case 5: launch_analyze<5>(*s, nblocks, mhd, s->an_part); break;
     default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
     }
     HIP_TRY(hipGetLastError());
@@ -711,6 +749,33 @@ int nemo_gpu_analyze(void* handle, double* sums) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(sums, s->an_out, nemo::AN_NSUMS * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return NEMO_OK;
+}
+
+int nemo_gpu_analyze_noalloc(void* handle, double* sums_dev) {
+    Solver* s = static_cast<Solver*>(handle);
+    if (int rc = check_ready(s)) return rc;
+    if (!sums_dev) return fail(NEMO_ERR_ARG, "null argument");
+    const long long nodes = s->nelem * (long long)s->M3;
+    if (nodes > NEMO_ANALYZE_NOALLOC_MAX_NODES)
+        return fail(NEMO_ERR_ARG, "nemo_gpu_analyze_noalloc: " + std::to_string(nodes) +
+                                      " nodes per variable exceed its scratch of " +
+                                      std::to_string((long long)NEMO_ANALYZE_NOALLOC_MAX_NODES) +
+                                      " (use nemo_gpu_analyze)");
+    if (!s->scratch) return fail(NEMO_ERR_NOT_READY, "scratch address not resolved");
+    const long long nblocks = (nodes + nemo::AN_NT - 1) / nemo::AN_NT;
+    const bool mhd = s->cfg.equations == NEMO_EQ_GLM_MHD;
+    // the same kernels as nemo_gpu_analyze; every partial row is written
+    // before it is read, so nothing needs clearing
+    switch (s->M) {
+    case 3: launch_analyze<3>(*s, nblocks, mhd, s->scratch); break;
+    case 4: launch_analyze<4>(*s, nblocks, mhd, s->scratch); break;
+    case 5: launch_analyze<5>(*s, nblocks, mhd, s->scratch); break;
+    default: return fail(NEMO_ERR_ARG, "n_poly must be 2, 3 or 4");
+    }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(nemo::analyze_finalize, dim3(1), dim3(nemo::AN_NT), 0, s->stream, s->scratch, nblocks, sums_dev);
+    HIP_TRY(hipGetLastError());
     return NEMO_OK;
 }
 

@@ -110,6 +110,14 @@ interface
         integer(c_int)                    :: rc
     end function
 
+    !! allocation-free form: sums_dev is a device buffer of NEMO_ANALYZE_NSUMS
+    !! doubles, filled asynchronously on the handle's stream
+    function nemo_gpu_analyze_noalloc(handle, sums_dev) result(rc) bind(c, name='nemo_gpu_analyze_noalloc')
+        import :: c_int, c_ptr
+        type(c_ptr), value                :: handle, sums_dev
+        integer(c_int)                    :: rc
+    end function
+
     function nemo_gpu_upload_state(handle, u) result(rc) bind(c, name='nemo_gpu_upload_state')
         import :: c_int, c_ptr, c_double
         type(c_ptr), value         :: handle

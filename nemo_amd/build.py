@@ -29,7 +29,7 @@ def _run(cmd, cwd=ROOT):
     return out
 
 
-HIP_SOURCES = ["nemo_gpu.hip", "batch.hip"]
+HIP_SOURCES = ["nemo_gpu.hip", "stage_m3.hip", "stage_m4.hip", "stage_m5.hip", "batch.hip"]
 
 
 def build_hip(verbose: bool = False) -> str:

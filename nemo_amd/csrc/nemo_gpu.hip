@@ -2,8 +2,9 @@
 //
 // Host side of the MI355X spatial operator: operator construction, device
 // memory, the per-stage launch sequence and the error convention.  The device
-// work lives in dg_stage.hpp.  There is no CPU fallback: every compute entry
-// point runs HIP kernels and reports HIP errors.
+// work lives in dg_stage.hpp; the stage launches of each polynomial degree
+// are compiled apart (stage_launch.hpp, stage_m{3,4,5}.hip).  There is no CPU
+// fallback: every compute entry point runs HIP kernels and reports HIP errors.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,6 +17,7 @@
 
 #include "../../include/nemo_gpu.h"
 #include "host_ops.hpp"
+#include "solver.hpp"
 #include "dg_mortar.hpp"
 #include "dg_analyze.hpp"
 
@@ -53,58 +55,6 @@ hipError_t scratch_device_ptr(double** out) {
     return hipSuccess;
 }
 
-// ---------------------------------------------------------------------------
-// Solver
-// ---------------------------------------------------------------------------
-
-struct Solver {
-    nemo_gpu_config cfg{};
-    HostOps ops{};
-    nemo::RKCoeffs rk{};
-    int M = 0, M3 = 0, V = 5;
-    long long nelem = 0;
-    bool have_mesh = false, have_state = false;
-
-    double* u[2] = {nullptr, nullptr};
-    int cur = 0;
-    double* r0 = nullptr;
-    double* r1 = nullptr;
-    double* rhs = nullptr;     // RHS / scratch of state size
-    double* alpha = nullptr;   // [nelem]
-    int32_t* nbr = nullptr;
-    double* delta = nullptr;
-    long long nmortar = 0;             // non-conforming (2:1) faces
-    int32_t* mortars = nullptr;        // [nmortar][6]
-    double* mortar_flux = nullptr;     // [5*nmortar][5][M*M]
-    int* counter = nullptr;            // remap fallbacks
-    double* an_part = nullptr;         // analysis partials
-    long long an_cap = 0;
-    double* an_out = nullptr;          // [NEMO_ANALYZE_NSUMS]
-    double* scratch = nullptr;         // this device's `scratch` (allocation-free analysis)
-    double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
-    double* part = nullptr;    // per-block CFL partials [nblocks][4]
-    int* status = nullptr;
-    double* hist = nullptr;
-    long long hist_cap = 0;
-    hipStream_t stream = nullptr;
-
-    // learned indicator
-    float* mlp_p = nullptr;     // canonical parameters
-    float* mlp_inf = nullptr;   // inference layout
-    float* mlp_m = nullptr;     // Adam moments
-    float* mlp_v = nullptr;
-    unsigned long long* mlp_stats = nullptr;  // judged, overridden, fallback
-    double* alpha_step = nullptr;  // [nelem]
-    bool mlp_ready = false;
-    uint32_t step_counter = 0;
-    double sample_fraction = 0.05, judge_tol = 0.05;
-
-    bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    size_t ev_used = 0;
-
-    size_t state_doubles() const { return size_t(V) * size_t(nelem) * size_t(M3); }
-};
 
 void free_mesh_buffers(Solver& s) {
     (void)hipFree(s.u[0]);
@@ -126,134 +76,6 @@ void free_mesh_buffers(Solver& s) {
     s.nmortar = 0;
 }
 
-// Per-launch options of the stage kernel.
-struct LaunchOpts {
-    int refresh = 1;            // learned indicator: compute the factor of the step
-    float* feat_out = nullptr;  // training data: features of every element
-};
-
-template <int M, typename Real>
-nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out,
-                                   double* alpha_out, double* cfl_part, const LaunchOpts& o) {
-    nemo::StageArgs<M, Real> a{};
-    a.mlp = s.mlp_inf;
-    a.alpha_step = s.alpha_step;
-    a.feat_out = o.feat_out;
-    a.mlp_stats = s.mlp_stats;
-    a.alpha_refresh = o.refresh;
-    a.step_index = s.step_counter;
-    a.sample_thresh = s.sample_fraction >= 1.0 ? 0xFFFFFFFFu : uint32_t(s.sample_fraction * 4294967296.0);
-    a.judge_tol = s.judge_tol;
-    a.u_in = s.u[s.cur];
-    a.u_out = s.u[s.cur ^ 1];
-    a.r0 = s.r0;
-    a.r1 = s.r1;
-    a.rhs_out = rhs_out;
-    a.alpha_out = alpha_out;
-    a.nbr = s.nbr;
-    a.mortar_flux = s.mortar_flux;
-    a.delta = s.delta;
-    a.dt_dev = dt_dev;
-    a.glm = s.dt_dev + 3;
-    a.status = s.status;
-    a.cfl_part = cfl_part;
-    a.nelem = s.nelem;
-    a.dt = dt;
-    a.stage = stage;
-    a.timedisc = s.cfg.timedisc;
-    a.indicator = s.cfg.indicator;
-    a.repair = s.cfg.repair;
-    a.alpha_max = s.cfg.alpha_max;
-    a.alpha_min = s.cfg.alpha_min;
-    a.ind_threshold = 0.5 * std::pow(10.0, -1.8 * std::pow(double(M), 0.25));
-    a.ind_s = std::log((1.0 - 0.0001) / 0.0001);
-    a.gas = make_gas<Real>(s.cfg.gamma);
-    a.mhd = make_mhd<Real>(s.cfg.gamma, s.cfg.mu0);
-    a.ph64 = make_phys64(s.cfg);
-    a.op = make_table<M, Real>(s.ops);
-    a.rk = s.rk;
-    for (int i = 0; i < M; ++i) a.rk.w64[i] = s.ops.w[i];
-    return a;
-}
-
-template <int EQ, int M, int VOL, int SURF, typename Real>
-void launch_stage_t(Solver& s, const nemo::StageArgs<M, Real>& a) {
-    constexpr int BE = Tile<M>::BE;
-    using Sh = nemo::StageShape<M, BE>;
-    const long long nblocks = (s.nelem + BE - 1) / BE;
-    hipLaunchKernelGGL((nemo::stage_kernel<EQ, M, BE, VOL, SURF, Real>), dim3(unsigned(nblocks)), dim3(Sh::NT), 0,
-                       s.stream, a);
-}
-
-// Fluxes of the non-conforming faces of the stage input (before the stage kernel).
-template <int M, typename Real>
-void launch_mortar(Solver& s) {
-    nemo::MortarArgs<M, Real> a{};
-    a.u_in = s.u[s.cur];
-    a.flux = s.mortar_flux;
-    a.mortars = s.mortars;
-    a.nelem = s.nelem;
-    a.gas = make_gas<Real>(s.cfg.gamma);
-    a.ph64 = make_phys64(s.cfg);
-    mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
-    const dim3 grid(unsigned(s.nmortar)), block(nemo::MortarShape<M>::NT);
-    if (s.cfg.surface == NEMO_SURF_HLLC)
-        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_HLLC, Real>), grid, block, 0, s.stream, a);
-    else
-        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_RUSANOV, Real>), grid, block, 0, s.stream, a);
-}
-
-template <int M, typename Real>
-int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                    double* cfl_part, const LaunchOpts& o) {
-    const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
-    const int vol = s.cfg.volume, surf = s.cfg.surface;
-    if (s.nmortar > 0) launch_mortar<M, Real>(s);
-    if constexpr (M == 4) {
-        // learned indicator, and the training-data pass of any Euler scheme
-        if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {
-            constexpr int BE = Tile<M>::BE;
-            const long long nblocks = (s.nelem + BE - 1) / BE;
-            if (o.feat_out || (vol == nemo::VOL_EC && surf == nemo::SURF_HLLC))
-                hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real, true>),
-                                   dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
-            else if (vol == nemo::VOL_EC && surf == nemo::SURF_RUSANOV)
-                hipLaunchKernelGGL(
-                    (nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_RUSANOV, Real, true>),
-                    dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
-            else
-                return fail(NEMO_ERR_ARG, "the learned indicator is built for the EC volume flux with HLLC or Rusanov");
-            return NEMO_OK;
-        }
-    }
-#define NEMO_CASE(E_, V_, S_)                          \
-    if (s.cfg.equations == E_ && vol == V_ && surf == S_) { \
-        launch_stage_t<E_, M, V_, S_, Real>(s, a);     \
-        return NEMO_OK;                                \
-    }
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_WEAK, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_WEAK, nemo::SURF_HLLC)
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_CENTRAL, nemo::SURF_HLLC)
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_RUSANOV)
-    NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_HLLC)
-    if constexpr (M == 4) {
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_HLLD)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_RUSANOV)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_HLLD)
-    }
-#undef NEMO_CASE
-    return fail(NEMO_ERR_ARG, "unsupported equations/volume/surface/n_poly combination");
-}
-
-template <int M>
-int launch_stage_m(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
-                   double* cfl_part, const LaunchOpts& o) {
-    if (s.cfg.precision == 1)
-        return launch_stage_mr<M, float>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
-    return launch_stage_mr<M, double>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
-}
 
 int launch_stage(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
                  double* cfl_part = nullptr, const LaunchOpts& o = LaunchOpts()) {

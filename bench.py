@@ -400,9 +400,11 @@ def main():
         # the same FP32 operator with the plain indicator, for reference
         wpp, lpp, mspp, _, _, _, _ = gpu_run(m, u0, N, args.steps, 1, cfg_kw, precision=1)
         upp = gpu_replay(m, u0, N, dts, cfg_kw, precision=1)
-        chosen = {"form": ("FP32 operator + FP64 RK/CFL/repair, learned MLP indicator (PP judge on 5%/step), "
-                           "per-stage launches") if params is not None else
-                          "FP32 operator + FP64 RK/CFL/repair, Persson-Peraire indicator (no trained network found)",
+        chosen = {"form": ("FP32 operator and LSRK register, FP64 state/update/CFL/repair, learned MLP indicator "
+                           "at the first stage of each step (PP judge on 5%/step), stored factor at stages 2-5")
+                          if params is not None else
+                          "FP32 operator and LSRK register, FP64 state/update/CFL/repair, Persson-Peraire indicator "
+                          "(no trained network found)",
                   "value": v32, "unit": "DOF-updates/s", "ms_per_step": wall32 / args.steps * 1e3,
                   "l1_density_err_vs_fp64": err, "l1_bar": 1e-6, "meets_l1_bar": err <= 1e-6,
                   "speedup_vs_plain": v32 / value,

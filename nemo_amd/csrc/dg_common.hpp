@@ -15,6 +15,13 @@ enum { VOL_WEAK = 0, VOL_CENTRAL = 1, VOL_EC = 2 };
 enum { SURF_RUSANOV = 0, SURF_HLLC = 1, SURF_HLLD = 2 };
 enum { TD_LSRK54 = 0, TD_SSPRK54 = 1 };
 enum { IND_NONE = 0, IND_PP = 1, IND_MLP = 2 };
+// How a stage kernel obtains the blending factor (compile time):
+// IK_RUNTIME: none or Persson-Peraire by StageArgs::indicator;
+// IK_MLP: the learned indicator with its judge (first stage of a step,
+// diagnostics, training data); IK_STORED: the factor kept in alpha_step
+// by the step's first stage (stages 2..5 of the learned indicator), so
+// that those stages carry neither the network nor the Persson-Peraire code.
+enum { IK_RUNTIME = 0, IK_MLP = 1, IK_STORED = 2 };
 
 // Operator tables for M LGL nodes (built on the host, passed by value).
 template <int M, typename Real>

@@ -539,7 +539,7 @@ __device__ __forceinline__ double probe_lsrk(const StageArgs<M, Real>& a, long l
 
 // One block of BE elements through a whole stage (the body of the stage
 // kernels); lds holds StageLds::DOUBLES doubles.
-template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP>
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, int IK>
 __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const long long blk, double* lds) {
     using Sh = StageShape<M, BE>;
     using L = StageLds<EQ, M, BE, Real>;
@@ -636,10 +636,14 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     __syncthreads();
 
     Real alpha = Real(0);
-    if constexpr (MLP)
+    if constexpr (IK == IK_MLP) {
         alpha = indicator_mlp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64);
-    else if (a.indicator == IND_PP)
+    } else if constexpr (IK == IK_STORED) {
+        const long long e = e0 + tid / Sh::M2;
+        alpha = e < a.nelem ? Real(a.alpha_step[e]) : Real(0);
+    } else if (a.indicator == IND_PP) {
         alpha = indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64);
+    }
 
     // ---- three direction passes ----------------------------------------
     if constexpr (EQ == EQ_MHD) {
@@ -768,18 +772,27 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     const bool lsrk = a.timedisc == TD_LSRK54;
     if constexpr (COMPACT) {
         // FP64 state from HBM, new state in registers until the operator's
-        // LDS is free
+        // LDS is free.  With LSRK the FP32 operator keeps its register in
+        // FP32 (the r0 buffer read as floats, half the register traffic):
+        // reg = a_s r0 + f is formed in FP64 and rounded once, and the state
+        // advances by the rounded value, so the next stage continues from
+        // the register it stored; u and its update stay FP64.
+        float* const r0f = reinterpret_cast<float*>(a.r0);
         double st[V][Q];
 #pragma unroll
         for (int h = 0; h < V; ++h) {
             double uin[Q], reg[Q];
+            float reg32[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int idx = tid + q * NT;
                 const long long gi = h * EM3 + e0 * M3 + idx;
                 const bool ok = idx < nvalid_nodes;
                 uin[q] = ok ? a.u_in[gi] : 0.0;
-                reg[q] = (ok && !(lsrk && s == 1)) ? a.r0[gi] : 0.0;
+                if (lsrk)
+                    reg32[q] = (ok && s != 1) ? r0f[gi] : 0.0f;
+                else
+                    reg[q] = ok ? a.r0[gi] : 0.0;
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -787,8 +800,15 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 st[h][q] = 0.0;
                 if (idx < nvalid_nodes) {
                     const long long gi = h * EM3 + e0 * M3 + idx;
-                    st[h][q] = lsrk ? probe_lsrk(a, gi, s, dt, uin[q], reg[q], dudt(h, slot<M>(idx), uin[q]))
-                                    : probe(gi, uin[q], dudt(h, slot<M>(idx), uin[q]));
+                    const double f = dudt(h, slot<M>(idx), uin[q]);
+                    if (lsrk) {
+                        // low-storage/4-5/timedisc_probe_mod.f90:31-37
+                        const float rg = float(s == 1 ? f : a.rk.lsrk_a[s - 1] * double(reg32[q]) + f);
+                        r0f[gi] = rg;
+                        st[h][q] = uin[q] + a.rk.lsrk_c[s - 1] * dt * double(rg);
+                    } else {
+                        st[h][q] = probe(gi, uin[q], f);
+                    }
                 }
             }
         }
@@ -884,12 +904,12 @@ constexpr int stage_waves_per_eu() {
 }
 
 // One workgroup per block of BE elements.
-template <int EQ, int M, int BE, int VOL, int SURF, typename Real, bool MLP = false>
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, int IK = IK_RUNTIME>
 __global__ __launch_bounds__(BE * M * M) __attribute__((amdgpu_waves_per_eu(stage_waves_per_eu<EQ, M, Real>())))
 void stage_kernel(StageArgs<M, Real> a) {
     __shared__ double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
     const long long nblocks = (a.nelem + BE - 1) / BE;
-    stage_block<EQ, M, BE, VOL, SURF, Real, MLP>(a, xcd_block(nblocks), lds);
+    stage_block<EQ, M, BE, VOL, SURF, Real, IK>(a, xcd_block(nblocks), lds);
 }
 
 // Persistent form for the batched ensemble: a grid of gridDim.x workgroups
@@ -914,7 +934,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel_persistent(StageArgs<
             b = it * G + blockIdx.x;
         }
         if (b >= nblocks) continue;
-        stage_block<EQ, M, BE, VOL, SURF, Real, false>(a, b, lds);
+        stage_block<EQ, M, BE, VOL, SURF, Real, IK_RUNTIME>(a, b, lds);
         __syncthreads();
     }
 }

@@ -93,15 +93,27 @@ int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, doubl
         if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {
             constexpr int BE = Tile<M>::BE;
             const long long nblocks = (s.nelem + BE - 1) / BE;
-            if (o.feat_out || (vol == nemo::VOL_EC && surf == nemo::SURF_HLLC))
-                hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real, true>),
-                                   dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
-            else if (vol == nemo::VOL_EC && surf == nemo::SURF_RUSANOV)
-                hipLaunchKernelGGL(
-                    (nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_RUSANOV, Real, true>),
-                    dim3(unsigned(nblocks)), dim3(BE * M * M), 0, s.stream, a);
-            else
+            // the network runs where the factor is computed (first stage of
+            // a step, diagnostics, training data); stages 2..5 read it back
+            const bool net = o.feat_out || o.refresh || rhs_out;
+            const dim3 grid{unsigned(nblocks)}, block{unsigned(BE * M * M)};
+            if (o.feat_out || (vol == nemo::VOL_EC && surf == nemo::SURF_HLLC)) {
+                if (net)
+                    hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real,
+                                                           nemo::IK_MLP>), grid, block, 0, s.stream, a);
+                else
+                    hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_HLLC, Real,
+                                                           nemo::IK_STORED>), grid, block, 0, s.stream, a);
+            } else if (vol == nemo::VOL_EC && surf == nemo::SURF_RUSANOV) {
+                if (net)
+                    hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_RUSANOV,
+                                                           Real, nemo::IK_MLP>), grid, block, 0, s.stream, a);
+                else
+                    hipLaunchKernelGGL((nemo::stage_kernel<nemo::EQ_EULER, M, BE, nemo::VOL_EC, nemo::SURF_RUSANOV,
+                                                           Real, nemo::IK_STORED>), grid, block, 0, s.stream, a);
+            } else {
                 return fail(NEMO_ERR_ARG, "the learned indicator is built for the EC volume flux with HLLC or Rusanov");
+            }
             return NEMO_OK;
         }
     }

@@ -359,23 +359,32 @@ __device__ __forceinline__ void logmean2(double a, double b, double& lm, double&
 // FP32 log-mean.  The reference's log branch loses ~eps/|a/b-1| relative
 // accuracy near its switch |a-b| = 1e-4 (harmless in FP64, 6e-4 in FP32), so
 // the FP32 operator uses the series of Ismail & Roe (JCP 228, 2009) in
-// f = (a-b)/(a+b):  L = (a+b) / (2 sum_k f^2k/(2k+1)), eight terms for
-// f^2 < 0.1 (truncation < 1e-9), and the log form beyond, where
-// |log(a/b)| > 0.65.
+// f = (a-b)/(a+b):  L = (a+b) / (2 sum_k f^2k/(2k+1)), ten terms for
+// f^2 < 0.2 (ratios a/b up to 2.6; truncation below 5e-9), and the log form
+// beyond, where |log(a/b)| > 0.9.  The log form is evaluated only when some
+// lane of the wavefront needs it (a wavefront-uniform branch): in smooth
+// flow a pair then costs one reciprocal (two with the mean itself) instead
+// of five transcendental instructions, which issue at a quarter of the
+// VALU rate.
 __device__ __forceinline__ void logmean2(float a, float b, float& lm, float& inv_lm) {
-    // both forms evaluated and selected (straight-line code, as in FP64)
-    const float r = __builtin_amdgcn_rcpf(a + b);
+    const float s = a + b;
+    const float r = __builtin_amdgcn_rcpf(s);
     const float d = a - b;
     const float f = d * r;
     const float u = f * f;
-    const bool series = u < 0.1f;
+    const bool series = u < 0.2f;
     const float F = 1.0f + u * (1.0f / 3.0f + u * (1.0f / 5.0f + u * (1.0f / 7.0f + u * (1.0f / 9.0f +
-                    u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f)))))));
-    const float l = flog(a * __builtin_amdgcn_rcpf(b));
-    const float num = series ? 0.5f * (a + b) : d;
-    const float den = series ? F : l;
-    lm = num * __builtin_amdgcn_rcpf(den);
-    inv_lm = series ? 2.0f * F * r : l * __builtin_amdgcn_rcpf(d);
+                    u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f + u * (1.0f / 17.0f +
+                    u * (1.0f / 19.0f)))))))));
+    lm = 0.5f * s * __builtin_amdgcn_rcpf(F);
+    inv_lm = 2.0f * F * r;
+    if (__builtin_expect(__any(!series), 0)) {
+        const float l = flog(a * __builtin_amdgcn_rcpf(b));
+        if (!series) {
+            lm = d * __builtin_amdgcn_rcpf(l);
+            inv_lm = l * __builtin_amdgcn_rcpf(d);
+        }
+    }
 }
 
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:

@@ -245,10 +245,11 @@ int nemo_gpu_mlp_train(void *handle, int64_t n, const float *features, const flo
 /* ---- batched ensemble ----------------------------------------------------
  * Independent runs of one scheme advanced together (the reference runs one
  * mesh per process; this has no counterpart there).  Every run has its own
- * mesh, state, n_poly (2..4) and CFL time step; runs of equal n_poly share
- * one persistent launch per RK stage, and pairs of time steps are replayed
- * as a captured HIP graph.  Built for Euler with the EC volume flux, HLLC,
- * no indicator or Persson-Peraire; cfg->n_poly is not used. */
+ * mesh, state, n_poly (2..4) and CFL time step; by default one launch per
+ * RK stage covers the element blocks of every run and degree, and pairs of
+ * time steps are replayed as a captured HIP graph.  Built for Euler with the
+ * EC volume flux, HLLC, no indicator or Persson-Peraire; cfg->n_poly is not
+ * used. */
 int nemo_gpu_batch_create(const nemo_gpu_config *cfg, int32_t nruns, const int32_t *n_poly,
                           const int64_t *nelem, void **handle);
 int nemo_gpu_batch_destroy(void *handle);
@@ -258,14 +259,17 @@ int nemo_gpu_batch_download(void *handle, int32_t run, double *u);
 /* nsteps CFL-limited steps of every run; dt_hist[nsteps][nruns] may be NULL.
  * Synchronises; NEMO_ERR_STATE if a repair failed. */
 int nemo_gpu_batch_run(void *handle, int32_t nsteps, int32_t use_graph, double *dt_hist);
-/* Launch geometry: grid_per_cu workgroups per CU for the persistent stage
- * launches (0: one workgroup per block of elements) and the XCD-contiguous
- * block map on or off. */
-int nemo_gpu_batch_set_launch(void *handle, int32_t grid_per_cu, int32_t xcd_map);
+/* Launch geometry: one_launch = 1 runs each RK stage as one launch over the
+ * blocks of all degrees (order 0: degree 4, 3, 2 blocks in that order, the
+ * costliest first; 1: 2, 3, 4), 0 as one launch per degree; grid_per_cu
+ * workgroups per CU walking the blocks persistently (0: one workgroup per
+ * block of elements) and the XCD-contiguous block map on or off.  Default:
+ * one launch per stage, one workgroup per block, order 0. */
+int nemo_gpu_batch_set_launch(void *handle, int32_t grid_per_cu, int32_t xcd_map, int32_t one_launch, int32_t order);
 /* Times `steps` steps of every candidate geometry on the current states
  * (which are restored afterwards), keeps the fastest and reports it. */
-int nemo_gpu_batch_autotune(void *handle, int32_t steps, int32_t *grid_per_cu, int32_t *xcd_map,
-                            double *best_ms_per_step);
+int nemo_gpu_batch_autotune(void *handle, int32_t steps, int32_t *grid_per_cu, int32_t *xcd_map, int32_t *one_launch,
+                            int32_t *order, double *best_ms_per_step);
 
 /* Operators for polynomial degree N (host only, no device needed):
  * LGL nodes x[M], weights w[M] (utils/interpol_mod.f90:513-582),

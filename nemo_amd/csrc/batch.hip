@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "host_ops.hpp"
+#include "dg_batch.hpp"
 
 namespace {
 
@@ -56,10 +57,19 @@ struct Batch {
     long long hist_rows = 0;
     hipStream_t stream = nullptr;
     int ncu = 256;
-    int grid_per_cu = 0, xcd = 1;
+    // launch geometry: one launch per stage for all degrees (one_launch) or
+    // one per degree; grid_per_cu workgroups per CU walking the blocks
+    // (0: one workgroup per block); the XCD-contiguous map; the order of the
+    // degree groups in the one launch (0: M = 5, 4, 3, the costliest blocks
+    // first; 1: M = 3, 4, 5)
+    int grid_per_cu = 0, xcd = 1, one_launch = 1, order = 0;
     hipGraphExec_t graph = nullptr;
     int graph_cur[3] = {-1, -1, -1};
-    int graph_grid = -1, graph_xcd = -1;
+    int graph_grid = -1, graph_xcd = -1, graph_one = -1, graph_order = -1;
+    // stage arguments of the one launch for the ten stages of two steps
+    // (slot k: stage k % 5 + 1 from buffer parity k & 1), device memory
+    void* margs = nullptr;
+    int margs_order = -1;
 };
 
 constexpr int group_of(int M) { return M - 3; }
@@ -87,6 +97,7 @@ void free_batch(Batch* b) {
     (void)hipFree(b->status);
     (void)hipFree(b->step_ctr);
     (void)hipFree(b->hist);
+    (void)hipFree(b->margs);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -183,18 +194,83 @@ __global__ void batch_count(unsigned long long* ctr) {
     if (threadIdx.x == 0) *ctr = *ctr + 1;
 }
 
+// Arguments of the one launch for the ten stages of two time steps: slot k
+// is stage k % 5 + 1 reading buffer k & 1 (every stage flips all groups'
+// buffers, and a step has five stages, so two steps return to the start).
+template <typename Real>
+int upload_multi_args(Batch* b) {
+    std::vector<nemo::MultiStageArgs<Real>> h(10);
+    int cur0[3];
+    for (int i = 0; i < 3; ++i) cur0[i] = b->g[i].cur;
+    for (int k = 0; k < 10; ++k) {
+        for (int i = 0; i < 3; ++i) b->g[i].cur = k & 1;
+        const int st = k % 5 + 1;
+        auto& m = h[size_t(k)];
+        if (b->g[0].nelem) m.a3 = batch_args<3, Real>(b, b->g[0], st, st == 5);
+        if (b->g[1].nelem) m.a4 = batch_args<4, Real>(b, b->g[1], st, st == 5);
+        if (b->g[2].nelem) m.a5 = batch_args<5, Real>(b, b->g[2], st, st == 5);
+        for (int i = 0; i < 3; ++i) m.nblocks[i] = b->g[i].nblocks;
+        const int ord[2][3] = {{2, 1, 0}, {0, 1, 2}};
+        for (int i = 0; i < 3; ++i) m.order[i] = ord[b->order][i];
+    }
+    for (int i = 0; i < 3; ++i) b->g[i].cur = cur0[i];
+    // earlier launches on the stream may still read the old arguments
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!b->margs) HIP_TRY(hipMalloc(&b->margs, h.size() * sizeof(h[0])));
+    HIP_TRY(hipMemcpy(b->margs, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+    b->margs_order = b->order;
+    return NEMO_OK;
+}
+
+// Buffer parity shared by the non-empty groups (they flip together).
+int batch_parity(const Batch* b) {
+    for (const auto& gr : b->g)
+        if (gr.nelem) return gr.cur;
+    return 0;
+}
+
+// Device arguments of the one launch for the current order; called before
+// any launch sequence (never during graph capture).
+int prepare_launch(Batch* b) {
+    if (!b->one_launch || b->margs_order == b->order) return NEMO_OK;
+    return b->cfg.precision == 1 ? upload_multi_args<float>(b) : upload_multi_args<double>(b);
+}
+
 template <typename Real>
 int launch_step_t(Batch* b) {
-    for (int st = 1; st <= 5; ++st) {
-        for (int gi = 0; gi < 3; ++gi) {
-            Group& gr = b->g[gi];
-            if (gr.nelem == 0) continue;
-            switch (gr.M) {
-            case 3: launch_group_stage<3, Real>(b, gr, st, st == 5); break;
-            case 4: launch_group_stage<4, Real>(b, gr, st, st == 5); break;
-            default: launch_group_stage<5, Real>(b, gr, st, st == 5); break;
+    if (b->one_launch) {
+        // one launch per stage for the blocks of every degree
+        if (b->margs_order != b->order) return fail(NEMO_ERR_NOT_READY, "batch launch arguments not prepared");
+        const int k0 = batch_parity(b) ? 5 : 0;
+        long long total = 0;
+        for (const auto& gr : b->g) total += gr.nblocks;
+        for (int st = 1; st <= 5; ++st) {
+            const auto* pma = static_cast<const nemo::MultiStageArgs<Real>*>(b->margs) + (k0 + st - 1);
+            if (b->grid_per_cu == 0) {
+                hipLaunchKernelGGL((nemo::stage_kernel_multi<Real>), dim3(unsigned(total)), dim3(64), 0, b->stream,
+                                   pma);
+            } else {
+                long long grid = (long long)b->grid_per_cu * b->ncu;
+                if (grid > total) grid = total;
+                if (b->xcd) grid = (grid + 7) / 8 * 8;  // whole XCD rounds
+                hipLaunchKernelGGL((nemo::stage_kernel_multi_persistent<Real>), dim3(unsigned(grid)), dim3(64), 0,
+                                   b->stream, pma, b->xcd);
             }
-            gr.cur ^= 1;
+            for (auto& gr : b->g)
+                if (gr.nelem) gr.cur ^= 1;
+        }
+    } else {
+        for (int st = 1; st <= 5; ++st) {
+            for (int gi = 0; gi < 3; ++gi) {
+                Group& gr = b->g[gi];
+                if (gr.nelem == 0) continue;
+                switch (gr.M) {
+                case 3: launch_group_stage<3, Real>(b, gr, st, st == 5); break;
+                case 4: launch_group_stage<4, Real>(b, gr, st, st == 5); break;
+                default: launch_group_stage<5, Real>(b, gr, st, st == 5); break;
+                }
+                gr.cur ^= 1;
+            }
         }
     }
     // the dt of the next step goes to the next history row
@@ -228,7 +304,9 @@ int launch_cfl(Batch* b) {
 }
 
 bool graph_matches(const Batch* b) {
-    if (!b->graph || b->graph_grid != b->grid_per_cu || b->graph_xcd != b->xcd) return false;
+    if (!b->graph || b->graph_grid != b->grid_per_cu || b->graph_xcd != b->xcd || b->graph_one != b->one_launch ||
+        b->graph_order != b->order)
+        return false;
     for (int i = 0; i < 3; ++i)
         if (b->g[i].nelem && b->graph_cur[i] != b->g[i].cur) return false;
     return true;
@@ -257,11 +335,14 @@ int build_graph(Batch* b) {
     for (int i = 0; i < 3; ++i) b->graph_cur[i] = b->g[i].cur;
     b->graph_grid = b->grid_per_cu;
     b->graph_xcd = b->xcd;
+    b->graph_one = b->one_launch;
+    b->graph_order = b->order;
     return NEMO_OK;
 }
 
 int run_steps(Batch* b, int nsteps, bool use_graph) {
     if (nsteps <= 0) return NEMO_OK;
+    if (int rc = prepare_launch(b)) return rc;
     if (int rc = launch_cfl(b)) return rc;
     int it = 0;
     if (use_graph && nsteps >= 2) {
@@ -446,11 +527,14 @@ int nemo_gpu_batch_download(void* handle, int32_t run, double* u) {
     return NEMO_OK;
 }
 
-int nemo_gpu_batch_set_launch(void* handle, int32_t grid_per_cu, int32_t xcd_map) {
+int nemo_gpu_batch_set_launch(void* handle, int32_t grid_per_cu, int32_t xcd_map, int32_t one_launch, int32_t order) {
     Batch* b = static_cast<Batch*>(handle);
     if (!b || grid_per_cu < 0 || grid_per_cu > 64) return fail(NEMO_ERR_ARG, "grid_per_cu must be in [0, 64]");
+    if (order < 0 || order > 1) return fail(NEMO_ERR_ARG, "order must be 0 or 1");
     b->grid_per_cu = grid_per_cu;
     b->xcd = xcd_map ? 1 : 0;
+    b->one_launch = one_launch ? 1 : 0;
+    b->order = order;
     return NEMO_OK;
 }
 
@@ -483,8 +567,8 @@ int nemo_gpu_batch_run(void* handle, int32_t nsteps, int32_t use_graph, double* 
     return status_check(b);
 }
 
-int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, int32_t* xcd_map,
-                            double* best_ms_per_step) {
+int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, int32_t* xcd_map, int32_t* one_launch,
+                            int32_t* order, double* best_ms_per_step) {
     Batch* b = static_cast<Batch*>(handle);
     if (!b || steps <= 0) return fail(NEMO_ERR_ARG, "invalid arguments");
     // snapshot the states (the tuning steps must not advance the runs)
@@ -500,19 +584,33 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
     }
     const unsigned long long big = 1ull << 62;
     HIP_TRY(hipMemcpyAsync(b->step_ctr, &big, sizeof(big), hipMemcpyHostToDevice, b->stream));
-    const int grids[] = {0, 1, 2, 3, 4, 6, 8, 12, 16};
+    // candidates: one launch per stage (both group orders) or one per
+    // degree, each with one workgroup per block or a persistent grid of
+    // 1..8 workgroups per CU with and without the XCD map
+    const int grids[] = {0, 1, 2, 4, 8};
+    struct Cand { int one, order, grid, x; };
+    std::vector<Cand> cands;
+    for (int one = 1; one >= 0; --one)
+        for (int ord = 0; ord < (one ? 2 : 1); ++ord)
+            for (int gpc : grids)
+                for (int x = 0; x < 2; ++x)
+                    if (!(gpc == 0 && x == 1)) cands.push_back({one, ord, gpc, x});
     double best = 1e300;
-    int bg = b->grid_per_cu, bx = b->xcd;
+    int bg = b->grid_per_cu, bx = b->xcd, bone = b->one_launch, bord = b->order;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     int rc = NEMO_OK;
-    for (int gpc : grids) {
-        for (int x = 0; x < 2 && rc == NEMO_OK; ++x) {
-            if (gpc == 0 && x == 1) continue;  // the map applies to persistent grids
-            b->grid_per_cu = gpc;
-            b->xcd = x;
-            rc = launch_cfl(b);
+    for (const Cand& c : cands) {
+        if (rc != NEMO_OK) break;
+        {
+            b->grid_per_cu = c.grid;
+            b->xcd = c.x;
+            b->one_launch = c.one;
+            b->order = c.order;
+            const int gpc = c.grid, x = c.x;
+            rc = prepare_launch(b);
+            if (rc == NEMO_OK) rc = launch_cfl(b);
             if (rc == NEMO_OK) rc = launch_step(b);  // warm-up
             if (rc != NEMO_OK) break;
             (void)hipEventRecord(e0, b->stream);
@@ -526,6 +624,8 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
                 best = per;
                 bg = gpc;
                 bx = x;
+                bone = c.one;
+                bord = c.order;
             }
             // back to the snapshot
             for (int i = 0; i < 3; ++i) {
@@ -546,8 +646,12 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
     if (rc != NEMO_OK) return rc;
     b->grid_per_cu = bg;
     b->xcd = bx;
+    b->one_launch = bone;
+    b->order = bord;
     if (grid_per_cu) *grid_per_cu = bg;
     if (xcd_map) *xcd_map = bx;
+    if (one_launch) *one_launch = bone;
+    if (order) *order = bord;
     if (best_ms_per_step) *best_ms_per_step = best;
     return NEMO_OK;
 }

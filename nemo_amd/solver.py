@@ -382,11 +382,19 @@ class Batch:
                    dptr(hist) if record_dt else None)
         return hist[:nsteps] if record_dt else None
 
-    def set_launch(self, grid_per_cu: int, xcd_map: bool = True) -> None:
-        self._call("nemo_gpu_batch_set_launch", self._h, ctypes.c_int32(grid_per_cu), ctypes.c_int32(int(xcd_map)))
+    def set_launch(self, grid_per_cu: int, xcd_map: bool = True, one_launch: bool = True, order: int = 0) -> None:
+        """Launch geometry (nemo_gpu_batch_set_launch): one launch per stage
+        over all degrees or one per degree, persistent grid per CU (0: one
+        workgroup per block), XCD map, order of the degree groups."""
+        self._call("nemo_gpu_batch_set_launch", self._h, ctypes.c_int32(grid_per_cu), ctypes.c_int32(int(xcd_map)),
+                   ctypes.c_int32(int(one_launch)), ctypes.c_int32(order))
 
-    def autotune(self, steps: int = 2):
-        g, x, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+    def autotune(self, steps: int = 2) -> dict:
+        """Time every candidate launch geometry, keep the fastest; returns it
+        with its ms per step."""
+        g, x, one, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_double()
         self._call("nemo_gpu_batch_autotune", self._h, ctypes.c_int32(steps), ctypes.byref(g), ctypes.byref(x),
-                   ctypes.byref(ms))
-        return g.value, bool(x.value), ms.value
+                   ctypes.byref(one), ctypes.byref(order), ctypes.byref(ms))
+        return {"grid_per_cu": g.value, "xcd_map": bool(x.value), "one_launch": bool(one.value),
+                "order": order.value, "ms_per_step": ms.value}

@@ -50,11 +50,13 @@ def test_graph_and_eager_and_geometry_agree():
     meshes, npolys, states = _runs(seed=9)
     cfg = solver.make_config(N=3, precision=1, **KW)
     out = []
-    for use_graph, grid, xcd in ((False, 0, False), (True, 0, False), (True, 2, True), (False, 3, False)):
+    geoms = ((False, 0, False, True, 0), (True, 0, False, True, 0), (True, 2, True, True, 1), (False, 3, False, True, 0),
+             (True, 0, False, False, 0), (True, 2, True, False, 0), (False, 3, False, False, 0))
+    for use_graph, grid, xcd, one, order in geoms:
         with solver.Batch(cfg, meshes, npolys) as b:
             for r, u in enumerate(states):
                 b.upload(r, u)
-            b.set_launch(grid, xcd)
+            b.set_launch(grid, xcd, one, order)
             b.run(5, use_graph=use_graph)
             out.append([b.download(r) for r in range(len(states))])
     for o in out[1:]:
@@ -68,8 +70,8 @@ def test_autotune_restores_state():
     with solver.Batch(cfg, meshes, npolys) as b:
         for r, u in enumerate(states):
             b.upload(r, u)
-        grid, xcd, ms = b.autotune(steps=2)
-        print("autotuned", grid, xcd, ms)
+        tuned = b.autotune(steps=2)
+        print("autotuned", tuned)
         after = [b.download(r) for r in range(len(states))]
         b.run(3)
         ua = [b.download(r) for r in range(len(states))]

@@ -182,6 +182,7 @@ def ensemble(nruns: int, steps: int, seed: int):
             b.upload(r, setups.planar_riemann(m, npolys[r], rng))
         t_setup = time.perf_counter() - t0
         tuned = b.autotune(steps=2)
+        table = b.tune_table()
         b.run(2)  # warm-up (builds the graph)
         t1 = time.perf_counter()
         b.run(steps, use_graph=True)
@@ -192,6 +193,9 @@ def ensemble(nruns: int, steps: int, seed: int):
             "value": dof * NSTAGES * steps / wall, "unit": "DOF-updates/s", "steps": steps,
             "ms_per_step": wall / steps * 1e3,
             "autotuned": dict(tuned, ms_per_step=round(tuned["ms_per_step"], 3)),
+            "best_one_launch_ms_per_step": min(t["ms_per_step"] for t in table if t["one_launch"]),
+            "best_per_degree_ms_per_step": min(t["ms_per_step"] for t in table if not t["one_launch"]),
+            "tuning_table": table,
             "setup_s": round(t_setup, 1)}
 
 

@@ -270,6 +270,10 @@ int nemo_gpu_batch_set_launch(void *handle, int32_t grid_per_cu, int32_t xcd_map
  * (which are restored afterwards), keeps the fastest and reports it. */
 int nemo_gpu_batch_autotune(void *handle, int32_t steps, int32_t *grid_per_cu, int32_t *xcd_map, int32_t *one_launch,
                             int32_t *order, double *best_ms_per_step);
+/* Timings of the last autotuning: *rows candidates; for the first max_rows,
+ * geometry[4*i..4*i+3] = (one_launch, order, grid_per_cu, xcd_map) and
+ * ms[i] its time per step. */
+int nemo_gpu_batch_tune_table(void *handle, int32_t max_rows, int32_t *rows, int32_t *geometry, double *ms);
 
 /* Operators for polynomial degree N (host only, no device needed):
  * LGL nodes x[M], weights w[M] (utils/interpol_mod.f90:513-582),

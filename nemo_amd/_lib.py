@@ -93,6 +93,7 @@ SIGNATURES = {
     "nemo_gpu_batch_run": [_VP, ctypes.c_int32, ctypes.c_int32, _P],
     "nemo_gpu_batch_set_launch": [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     "nemo_gpu_batch_autotune": [_VP, ctypes.c_int32, _PI32, _PI32, _PI32, _PI32, _P],
+    "nemo_gpu_batch_tune_table": [_VP, ctypes.c_int32, _PI32, _PI32, _P],
     "nemo_gpu_set_timing": [_VP, ctypes.c_int32],
     "nemo_gpu_copy_calibration": [_VP, ctypes.c_int64],
     "nemo_gpu_check_arithmetic": [_VP, ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <vector>
 
@@ -70,6 +71,9 @@ struct Batch {
     // (slot k: stage k % 5 + 1 from buffer parity k & 1), device memory
     void* margs = nullptr;
     int margs_order = -1;
+    // timings of the last autotuning: (one_launch, order, grid_per_cu, xcd) -> ms per step
+    std::vector<std::array<int, 4>> tune_geom;
+    std::vector<double> tune_ms;
 };
 
 constexpr int group_of(int M) { return M - 3; }
@@ -597,6 +601,8 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
                     if (!(gpc == 0 && x == 1)) cands.push_back({one, ord, gpc, x});
     double best = 1e300;
     int bg = b->grid_per_cu, bx = b->xcd, bone = b->one_launch, bord = b->order;
+    b->tune_geom.clear();
+    b->tune_ms.clear();
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -620,6 +626,8 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
             const double per = double(ms) / steps;
+            b->tune_geom.push_back({c.one, c.order, c.grid, c.x});
+            b->tune_ms.push_back(per);
             if (per < best) {
                 best = per;
                 bg = gpc;
@@ -653,6 +661,19 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
     if (one_launch) *one_launch = bone;
     if (order) *order = bord;
     if (best_ms_per_step) *best_ms_per_step = best;
+    return NEMO_OK;
+}
+
+int nemo_gpu_batch_tune_table(void* handle, int32_t max_rows, int32_t* rows, int32_t* geometry, double* ms) {
+    Batch* b = static_cast<Batch*>(handle);
+    if (!b || max_rows < 0 || !rows) return fail(NEMO_ERR_ARG, "invalid arguments");
+    const int n = int(b->tune_ms.size());
+    *rows = n;
+    for (int i = 0; i < n && i < max_rows; ++i) {
+        if (geometry)
+            for (int k = 0; k < 4; ++k) geometry[4 * i + k] = b->tune_geom[size_t(i)][size_t(k)];
+        if (ms) ms[i] = b->tune_ms[size_t(i)];
+    }
     return NEMO_OK;
 }
 

@@ -72,7 +72,7 @@ constexpr int multi_waves_per_eu() {
 template <typename Real>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(multi_waves_per_eu<Real>())))
 void stage_kernel_multi(const MultiStageArgs<Real>* __restrict__ pma) {
-    __shared__ double lds[MultiLds<Real>::DOUBLES];
+    __shared__ __align__(16) double lds[MultiLds<Real>::DOUBLES];
     const MultiStageArgs<Real>& ma = *pma;
     long long w = blockIdx.x;
     int g = ma.order[0];
@@ -94,7 +94,7 @@ void stage_kernel_multi(const MultiStageArgs<Real>* __restrict__ pma) {
 template <typename Real>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(multi_waves_per_eu<Real>())))
 void stage_kernel_multi_persistent(const MultiStageArgs<Real>* __restrict__ pma, int xcd) {
-    __shared__ double lds[MultiLds<Real>::DOUBLES];
+    __shared__ __align__(16) double lds[MultiLds<Real>::DOUBLES];
     const MultiStageArgs<Real>& ma = *pma;
     const long long G = gridDim.x, w = blockIdx.x;
     for (int k = 0; k < 3; ++k) {

@@ -286,7 +286,8 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
     for (int k = 0; k < BE; ++k)
 #pragma unroll
         for (int f = 0; f < 4; ++f) feat[k][f] = lane_bcast(x[f], k * 16);
-    mlp_forward_wave<BE>(a.mlp, feat, y);
+    // the block's accumulator rows are free until the direction passes
+    mlp_forward_wave<BE>(a.mlp, feat, y, reinterpret_cast<float*>(s_acc[0]));
     float ym = y[0];
 #pragma unroll
     for (int k = 1; k < BE; ++k)
@@ -907,7 +908,7 @@ constexpr int stage_waves_per_eu() {
 template <int EQ, int M, int BE, int VOL, int SURF, typename Real, int IK = IK_RUNTIME>
 __global__ __launch_bounds__(BE * M * M) __attribute__((amdgpu_waves_per_eu(stage_waves_per_eu<EQ, M, Real>())))
 void stage_kernel(StageArgs<M, Real> a) {
-    __shared__ double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
+    __shared__ __align__(16) double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
     const long long nblocks = (a.nelem + BE - 1) / BE;
     stage_block<EQ, M, BE, VOL, SURF, Real, IK>(a, xcd_block(nblocks), lds);
 }
@@ -918,7 +919,7 @@ void stage_kernel(StageArgs<M, Real> a) {
 // stretch of blocks.
 template <int EQ, int M, int BE, int VOL, int SURF, typename Real>
 __global__ __launch_bounds__(BE * M * M) void stage_kernel_persistent(StageArgs<M, Real> a, int xcd) {
-    __shared__ double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
+    __shared__ __align__(16) double lds[StageLds<EQ, M, BE, Real>::DOUBLES];
     const long long nblocks = (a.nelem + BE - 1) / BE;
     const long long G = gridDim.x;
     const long long per = (nblocks + G - 1) / G;

@@ -398,3 +398,14 @@ class Batch:
                    ctypes.byref(one), ctypes.byref(order), ctypes.byref(ms))
         return {"grid_per_cu": g.value, "xcd_map": bool(x.value), "one_launch": bool(one.value),
                 "order": order.value, "ms_per_step": ms.value}
+
+    def tune_table(self) -> list:
+        """Every candidate of the last autotuning with its ms per step."""
+        n = ctypes.c_int32()
+        self._call("nemo_gpu_batch_tune_table", self._h, ctypes.c_int32(0), ctypes.byref(n), None, None)
+        geo = np.zeros((max(n.value, 1), 4), dtype=np.int32)
+        ms = np.zeros(max(n.value, 1))
+        self._call("nemo_gpu_batch_tune_table", self._h, ctypes.c_int32(n.value), ctypes.byref(n), iptr(geo),
+                   dptr(ms))
+        return [{"one_launch": bool(g[0]), "order": int(g[1]), "grid_per_cu": int(g[2]), "xcd_map": bool(g[3]),
+                 "ms_per_step": round(float(t), 3)} for g, t in zip(geo[:n.value], ms[:n.value])]

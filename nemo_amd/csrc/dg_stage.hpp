@@ -286,8 +286,7 @@ __device__ __forceinline__ Real indicator_mlp(const StageArgs<M, Real>& a, long 
     for (int k = 0; k < BE; ++k)
 #pragma unroll
         for (int f = 0; f < 4; ++f) feat[k][f] = lane_bcast(x[f], k * 16);
-    // the block's accumulator rows are free until the direction passes
-    mlp_forward_wave<BE>(a.mlp, feat, y, reinterpret_cast<float*>(s_acc[0]));
+    mlp_forward_wave<BE>(a.mlp, feat, y);
     float ym = y[0];
 #pragma unroll
     for (int k = 1; k < BE; ++k)

@@ -58,44 +58,12 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// One 64 x 64 hidden layer for NB samples: lane j forms unit j,
-// z[s] = b_j + sum_k W[j][k] act_k(s).  The activations of the previous layer
-// (lane k holds act_k) are passed through LDS (act, NB x 64 floats) and read
-// back four at a time by every lane from one address (a broadcast read, no
-// bank conflicts), so the layer costs the wavefront 64 NB FMAs per lane and
-// 16 NB LDS reads instead of a v_readlane per product.
-template <int NB>
-__device__ __forceinline__ void mlp_layer_lds(const float* __restrict__ W, float b, const float (&in)[NB],
-                                              float (&out)[NB], float* act) {
-    const int j = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 0; s < NB; ++s) act[s * MLP_H + j] = in[s];
-    __syncthreads();
-    float z[NB];
-#pragma unroll
-    for (int s = 0; s < NB; ++s) z[s] = b;
-    for (int k = 0; k < MLP_H; k += 4) {
-        const float w0 = W[(k + 0) * MLP_H + j], w1 = W[(k + 1) * MLP_H + j];  // transposed: W[j][k]
-        const float w2 = W[(k + 2) * MLP_H + j], w3 = W[(k + 3) * MLP_H + j];
-#pragma unroll
-        for (int s = 0; s < NB; ++s) {
-            const float4 x = *reinterpret_cast<const float4*>(&act[s * MLP_H + k]);
-            z[s] += w0 * x.x + w1 * x.y + w2 * x.z + w3 * x.w;
-        }
-    }
-    __syncthreads();  // act is overwritten by the next layer
-#pragma unroll
-    for (int s = 0; s < NB; ++s) out[s] = gelu(z[s]);
-}
-
 // Forward pass for NB samples held uniformly by the wavefront (feat[s][k] is
 // the same in every lane), with the inference-layout parameters P.  Lane j
 // computes unit j of every hidden layer; y[s] (uniform) is the sigmoid output.
-// With act (NB x 64 floats of LDS, 16-byte aligned) the hidden layers pass
-// their activations through it (mlp_layer_lds); without, by v_readlane.
 template <int NB>
 __device__ __forceinline__ void mlp_forward_wave(const float* __restrict__ P, const float (&feat)[NB][MLP_IN],
-                                                 float (&y)[NB], float* act = nullptr) {
+                                                 float (&y)[NB]) {
     const int j = threadIdx.x & 63;
     float a1[NB], a2[NB], a3[NB];
     {
@@ -105,36 +73,31 @@ __device__ __forceinline__ void mlp_forward_wave(const float* __restrict__ P, co
         for (int s = 0; s < NB; ++s)
             a1[s] = gelu(b + w0 * feat[s][0] + w1 * feat[s][1] + w2 * feat[s][2] + w3 * feat[s][3]);
     }
-    if (act) {
-        mlp_layer_lds<NB>(P + MLP_W2, P[MLP_B2 + j], a1, a2, act);
-        mlp_layer_lds<NB>(P + MLP_W3, P[MLP_B3 + j], a2, a3, act);
-    } else {
-        {
-            float z[NB];
-            const float b = P[MLP_B2 + j];
+    {
+        float z[NB];
+        const float b = P[MLP_B2 + j];
 #pragma unroll
-            for (int s = 0; s < NB; ++s) z[s] = b;
-            for (int k = 0; k < MLP_H; ++k) {
-                const float w = P[MLP_W2 + k * MLP_H + j];  // transposed: W2[j][k]
+        for (int s = 0; s < NB; ++s) z[s] = b;
+        for (int k = 0; k < MLP_H; ++k) {
+            const float w = P[MLP_W2 + k * MLP_H + j];  // transposed: W2[j][k]
 #pragma unroll
-                for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a1[s], k);
-            }
-#pragma unroll
-            for (int s = 0; s < NB; ++s) a2[s] = gelu(z[s]);
+            for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a1[s], k);
         }
-        {
-            float z[NB];
-            const float b = P[MLP_B3 + j];
 #pragma unroll
-            for (int s = 0; s < NB; ++s) z[s] = b;
-            for (int k = 0; k < MLP_H; ++k) {
-                const float w = P[MLP_W3 + k * MLP_H + j];
+        for (int s = 0; s < NB; ++s) a2[s] = gelu(z[s]);
+    }
+    {
+        float z[NB];
+        const float b = P[MLP_B3 + j];
 #pragma unroll
-                for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a2[s], k);
-            }
+        for (int s = 0; s < NB; ++s) z[s] = b;
+        for (int k = 0; k < MLP_H; ++k) {
+            const float w = P[MLP_W3 + k * MLP_H + j];
 #pragma unroll
-            for (int s = 0; s < NB; ++s) a3[s] = gelu(z[s]);
+            for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a2[s], k);
         }
+#pragma unroll
+        for (int s = 0; s < NB; ++s) a3[s] = gelu(z[s]);
     }
     const float w4 = P[MLP_W4 + j], b4 = P[MLP_B4];
 #pragma unroll

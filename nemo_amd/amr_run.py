@@ -80,16 +80,27 @@ class AdaptiveRun:
     dof_updates: int = 0        # element-node updates summed over steps (times stages)
     elements: list = field(default_factory=list)
     remesh_seconds: float = 0.0
+    # the indicator masks of the last pass that left the forest unchanged:
+    # the flags, and so the outcome, are a function of the forest and these
+    # masks, so an equal pair on the same forest is steady again
+    _steady: tuple = field(default=None, repr=False)
 
     def adapt(self) -> np.ndarray | None:
         t0 = time.perf_counter()
         alpha = self.solver.indicator()
+        masks = (alpha > self.refine_above, alpha <= self.coarsen_below)
+        if self._steady is not None and all(np.array_equal(m, k) for m, k in zip(masks, self._steady)):
+            self.remesh_seconds += time.perf_counter() - t0
+            return None
         refine, coarsen = adapt_flags(self.forest, alpha, self.refine_above, self.coarsen_below)
         new, units = self.forest.adapt(refine, coarsen)
         if units is not None:
             self.fallbacks += self.solver.remesh(new, units)
             self.forest = new
             self.remeshes += 1
+            self._steady = None
+        else:
+            self._steady = masks
         self.remesh_seconds += time.perf_counter() - t0
         return units
 

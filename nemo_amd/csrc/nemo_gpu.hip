@@ -69,6 +69,7 @@ void free_mesh_buffers(Solver& s) {
     (void)hipFree(s.alpha_step);
     s.u[0] = s.u[1] = s.r0 = s.r1 = s.rhs = s.alpha = s.delta = s.part = s.alpha_step = nullptr;
     s.nbr = nullptr;
+    s.cap = 0;
     (void)hipFree(s.mortars);
     (void)hipFree(s.mortar_flux);
     s.mortars = nullptr;
@@ -242,19 +243,27 @@ int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, 
     s.have_mesh = false;
     if (nelem != s.nelem) {
         s.have_state = false;
-        free_mesh_buffers(s);
+        if (nelem > s.cap) {
+            // mesh-sized buffers for this mesh and later ones up to 5/4 of it
+            // (adaptation changes the element count a little at a time)
+            free_mesh_buffers(s);
+            const long long cap = nelem + nelem / 4;
+            s.nelem = cap;  // sizes below are for the capacity
+            const size_t bytes = s.state_doubles() * sizeof(double);
+            HIP_TRY(hipMalloc(&s.u[0], bytes));
+            HIP_TRY(hipMalloc(&s.u[1], bytes));
+            HIP_TRY(hipMalloc(&s.r0, bytes));
+            HIP_TRY(hipMalloc(&s.r1, bytes));
+            HIP_TRY(hipMalloc(&s.rhs, bytes));
+            HIP_TRY(hipMalloc(&s.alpha, size_t(cap) * sizeof(double)));
+            HIP_TRY(hipMalloc(&s.nbr, size_t(cap) * 6 * sizeof(int32_t)));
+            HIP_TRY(hipMalloc(&s.delta, size_t(cap) * 3 * sizeof(double)));
+            HIP_TRY(hipMalloc(&s.part, size_t(nblocks_for(s)) * 4 * sizeof(double)));
+            HIP_TRY(hipMalloc(&s.alpha_step, size_t(cap) * sizeof(double)));
+            s.cap = cap;
+        }
         s.nelem = nelem;
         const size_t bytes = s.state_doubles() * sizeof(double);
-        HIP_TRY(hipMalloc(&s.u[0], bytes));
-        HIP_TRY(hipMalloc(&s.u[1], bytes));
-        HIP_TRY(hipMalloc(&s.r0, bytes));
-        HIP_TRY(hipMalloc(&s.r1, bytes));
-        HIP_TRY(hipMalloc(&s.rhs, bytes));
-        HIP_TRY(hipMalloc(&s.alpha, size_t(nelem) * sizeof(double)));
-        HIP_TRY(hipMalloc(&s.nbr, size_t(nelem) * 6 * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&s.delta, size_t(nelem) * 3 * sizeof(double)));
-        HIP_TRY(hipMalloc(&s.part, size_t(nblocks_for(s)) * 4 * sizeof(double)));
-        HIP_TRY(hipMalloc(&s.alpha_step, size_t(nelem) * sizeof(double)));
         HIP_TRY(hipMemsetAsync(s.alpha_step, 0, size_t(nelem) * sizeof(double), s.stream));
         HIP_TRY(hipMemsetAsync(s.r0, 0, bytes, s.stream));
         HIP_TRY(hipMemsetAsync(s.r1, 0, bytes, s.stream));
@@ -412,6 +421,7 @@ int nemo_gpu_destroy(void* handle) {
     (void)hipFree(s->dt_dev);
     (void)hipFree(s->status);
     (void)hipFree(s->counter);
+    (void)hipFree(s->units);
     (void)hipFree(s->an_part);
     (void)hipFree(s->an_out);
     (void)hipFree(s->mlp_p);
@@ -464,13 +474,23 @@ int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const doubl
     if (io != s->nelem || jn != nelem) return fail(NEMO_ERR_ARG, "replacement units do not cover both meshes");
 
     const size_t new_bytes = size_t(s->V) * size_t(nelem) * size_t(s->M3) * sizeof(double);
-    DeviceBuffer u_new, d_units;  // freed on every early return
-    HIP_TRY(hipMalloc(&u_new.p, new_bytes));
-    HIP_TRY(hipMalloc(&d_units.p, size_t(nunits) * 3 * sizeof(int32_t)));
-    HIP_TRY(hipMemcpyAsync(d_units.p, units, size_t(nunits) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    // The new state goes to the idle ping-pong buffer when the new mesh fits
+    // the buffers' capacity (no allocation); otherwise to a buffer of its own,
+    // copied into the regrown state buffer once the mesh is set.
+    const bool fits = nelem <= s->cap;
+    DeviceBuffer u_new;  // freed on every early return
+    if (!fits) HIP_TRY(hipMalloc(&u_new.p, new_bytes));
+    if (nunits > s->units_cap) {
+        (void)hipFree(s->units);
+        s->units = nullptr;
+        s->units_cap = 0;
+        HIP_TRY(hipMalloc(&s->units, size_t(nunits) * 3 * sizeof(int32_t)));
+        s->units_cap = nunits;
+    }
+    HIP_TRY(hipMemcpyAsync(s->units, units, size_t(nunits) * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->counter, 0, sizeof(int), s->stream));
-    double* un = static_cast<double*>(u_new.p);
-    const int32_t* du = static_cast<const int32_t*>(d_units.p);
+    double* un = fits ? s->u[s->cur ^ 1] : static_cast<double*>(u_new.p);
+    const int32_t* du = s->units;
     switch (s->M) {
     case 3: launch_remap<3>(*s, un, du, nunits, nelem); break;
     case 4: launch_remap<4>(*s, un, du, nunits, nelem); break;
@@ -482,10 +502,14 @@ int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const doubl
     HIP_TRY(hipMemcpyAsync(&nfb, s->counter, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (fallbacks) *fallbacks = nfb;
-    if (int rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
-    (void)hipFree(s->u[s->cur]);
-    s->u[s->cur] = un;
-    u_new.p = nullptr;  // owned by the solver now
+    if (fits) {
+        if (int rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
+        s->cur ^= 1;
+    } else {
+        // set_mesh regrows every mesh-sized buffer (the old state is dead now)
+        if (int rc = set_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
+        HIP_TRY(hipMemcpyAsync(s->u[s->cur], un, new_bytes, hipMemcpyDeviceToDevice, s->stream));
+    }
     HIP_TRY(hipMemsetAsync(s->r0, 0, new_bytes, s->stream));
     HIP_TRY(hipMemsetAsync(s->r1, 0, new_bytes, s->stream));
     HIP_TRY(hipMemsetAsync(s->alpha_step, 0, size_t(nelem) * sizeof(double), s->stream));

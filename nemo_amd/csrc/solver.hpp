@@ -20,6 +20,7 @@ struct Solver {
     nemo::RKCoeffs rk{};
     int M = 0, M3 = 0, V = 5;
     long long nelem = 0;
+    long long cap = 0;                 // elements the mesh-sized buffers hold (>= nelem)
     bool have_mesh = false, have_state = false;
 
     double* u[2] = {nullptr, nullptr};
@@ -34,6 +35,8 @@ struct Solver {
     int32_t* mortars = nullptr;        // [nmortar][6]
     double* mortar_flux = nullptr;     // [5*nmortar][5][M*M]
     int* counter = nullptr;            // remap fallbacks
+    int32_t* units = nullptr;          // replacement units of the last remesh
+    long long units_cap = 0;
     double* an_part = nullptr;         // analysis partials
     long long an_cap = 0;
     double* an_out = nullptr;          // [NEMO_ANALYZE_NSUMS]

@@ -589,9 +589,10 @@ int nemo_gpu_batch_autotune(void* handle, int32_t steps, int32_t* grid_per_cu, i
     const unsigned long long big = 1ull << 62;
     HIP_TRY(hipMemcpyAsync(b->step_ctr, &big, sizeof(big), hipMemcpyHostToDevice, b->stream));
     // candidates: one launch per stage (both group orders) or one per
-    // degree, each with one workgroup per block or a persistent grid of
-    // 1..8 workgroups per CU with and without the XCD map
-    const int grids[] = {0, 1, 2, 4, 8};
+    // degree, each with one workgroup per block or a persistent grid of 8
+    // workgroups per CU with and without the XCD map (smaller persistent
+    // grids leave the CUs underoccupied: 2-4x slower when measured)
+    const int grids[] = {0, 8};
     struct Cand { int one, order, grid, x; };
     std::vector<Cand> cands;
     for (int one = 1; one >= 0; --one)

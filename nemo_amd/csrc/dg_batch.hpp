@@ -92,8 +92,8 @@ void stage_kernel_multi(const MultiStageArgs<Real>* __restrict__ pma) {
 // XCD w % 8) takes the blocks of its XCD's contiguous eighth of each group.
 // All loop bounds are uniform, so every wave leaves every loop.
 template <typename Real>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(multi_waves_per_eu<Real>())))
-void stage_kernel_multi_persistent(const MultiStageArgs<Real>* __restrict__ pma, int xcd) {
+__global__ __launch_bounds__(64) void stage_kernel_multi_persistent(const MultiStageArgs<Real>* __restrict__ pma,
+                                                                    int xcd) {
     __shared__ __align__(16) double lds[MultiLds<Real>::DOUBLES];
     const MultiStageArgs<Real>& ma = *pma;
     const long long G = gridDim.x, w = blockIdx.x;

@@ -387,6 +387,26 @@ __device__ __forceinline__ void logmean2(float a, float b, float& lm, float& inv
     }
 }
 
+// FP32 log-mean with both forms evaluated and selected (straight-line code):
+// the same series (eight terms for f^2 < 0.1, truncation < 1e-9) and log
+// form as logmean2, for the GLM-MHD direction pass, where the branch of
+// logmean2 raises the register pressure past the register file (1 KB of
+// scratch per lane, 2.6x slower).
+__device__ __forceinline__ void logmean2_select(float a, float b, float& lm, float& inv_lm) {
+    const float r = __builtin_amdgcn_rcpf(a + b);
+    const float d = a - b;
+    const float f = d * r;
+    const float u = f * f;
+    const bool series = u < 0.1f;
+    const float F = 1.0f + u * (1.0f / 3.0f + u * (1.0f / 5.0f + u * (1.0f / 7.0f + u * (1.0f / 9.0f +
+                    u * (1.0f / 11.0f + u * (1.0f / 13.0f + u * (1.0f / 15.0f)))))));
+    const float l = flog(a * __builtin_amdgcn_rcpf(b));
+    const float num = series ? 0.5f * (a + b) : d;
+    const float den = series ? F : l;
+    lm = num * __builtin_amdgcn_rcpf(den);
+    inv_lm = series ? 2.0f * F * r : l * __builtin_amdgcn_rcpf(d);
+}
+
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:
 // two_point_flux_mod.f90:34-77, rotated to direction D (:79-141).
 template <int D, typename Real>

@@ -165,7 +165,7 @@ __device__ __forceinline__ void mhd_nc_add(const MhdGas<Real>& g, const MhdState
 }
 
 // LN_MEAN: equations/mhd/two_point/ec/two_point_flux_mod.f90:203-219, with
-// its reciprocal.  FP32: the Ismail-Roe series form of logmean2.
+// its reciprocal.  FP32: the Ismail-Roe series form (logmean2_select).
 // Both branches are evaluated and selected (straight-line code, as the
 // Euler log-mean); Xi takes the IEEE-rounded bits (div_rn), which the log
 // branch amplifies by 1/|Xi - 1|.
@@ -182,7 +182,9 @@ __device__ __forceinline__ void ln_mean(double aL, double aR, double& lm, double
     lm = num * rcp_nr(den);
     inv_lm = den * rcp_nr(num);
 }
-__device__ __forceinline__ void ln_mean(float aL, float aR, float& lm, float& inv_lm) { logmean2(aL, aR, lm, inv_lm); }
+__device__ __forceinline__ void ln_mean(float aL, float aR, float& lm, float& inv_lm) {
+    logmean2_select(aL, aR, lm, inv_lm);
+}
 
 // Entropy-conserving GLM-MHD flux of Derigs et al. (JCP 364, 2018):
 // equations/mhd/two_point/ec/two_point_flux_mod.f90:13-86, rotated to D

@@ -182,7 +182,7 @@ def ensemble(nruns: int, steps: int, seed: int):
             b.upload(r, setups.planar_riemann(m, npolys[r], rng))
         t_setup = time.perf_counter() - t0
         tuned = b.autotune(steps=2)
-        table = b.tune_table()
+        table = [dict(t, ms_per_step=round(t["ms_per_step"], 3)) for t in b.tune_table()]
         b.run(2)  # warm-up (builds the graph)
         t1 = time.perf_counter()
         b.run(steps, use_graph=True)

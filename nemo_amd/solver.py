@@ -408,4 +408,4 @@ class Batch:
         self._call("nemo_gpu_batch_tune_table", self._h, ctypes.c_int32(n.value), ctypes.byref(n), iptr(geo),
                    dptr(ms))
         return [{"one_launch": bool(g[0]), "order": int(g[1]), "grid_per_cu": int(g[2]), "xcd_map": bool(g[3]),
-                 "ms_per_step": round(float(t), 3)} for g, t in zip(geo[:n.value], ms[:n.value])]
+                 "ms_per_step": float(t)} for g, t in zip(geo[:n.value], ms[:n.value])]

@@ -149,6 +149,39 @@ def test_remesh_matches_oracle(N):
     assert _rel(r, O.rhs(o, g.nbr, g.delta, ug, O.indicator(o, ug)[0])) <= RHS_TOL
 
 
+def test_remesh_chain_grow_then_fit():
+    """Three state transfers in a row on one solver: a refinement past the
+    mesh buffers' capacity (the buffers regrow), a coarsening back (the new
+    state goes to the idle buffer, nothing is allocated) and a second
+    refinement that fits.  Each transferred state equals the oracle's
+    O.remap of the previous one (<= 1e-14) and the RHS on the last mesh
+    matches the oracle."""
+    N = 3
+    f0 = amr.Forest.uniform(2, minlevel=2, maxlevel=4)
+    u = _state(f0, N)
+    cen = lambda f: f.lower + 0.5 * f.delta
+    c0 = np.array([0.45, 0.4, 0.5])
+    g1, units1 = f0.adapt(np.linalg.norm(cen(f0) - c0, axis=1) < 0.4, np.zeros(f0.nelem, bool))
+    assert g1.nelem > 1.25 * f0.nelem   # past the capacity of the first upload
+    g2, units2 = g1.adapt(np.zeros(g1.nelem, bool), np.linalg.norm(cen(g1) - c0, axis=1) > 0.2)
+    assert g2.nelem < g1.nelem
+    g3, units3 = g2.adapt(np.linalg.norm(cen(g2) - c0, axis=1) < 0.15, np.zeros(g2.nelem, bool))
+    assert g2.nelem < g3.nelem <= g1.nelem
+    print(f"remesh chain: {f0.nelem} -> {g1.nelem} -> {g2.nelem} -> {g3.nelem} elements", flush=True)
+    cfg, o = _cfgs(g3, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, repair=1)
+    with solver.Solver(cfg) as s:
+        s.upload_mesh(f0)
+        s.set_state(u)
+        for g, units in ((g1, units1), (g2, units2), (g3, units3)):
+            s.remesh(g, units)
+            ug = s.get_state()
+            uref, _ = O.remap(N, 1.4, u, units, g.nelem)
+            assert _rel(ug, uref) <= 1e-14
+            u = uref
+        r = s.kernel_calc_flux()
+    assert _rel(r, O.rhs(o, g3.nbr, g3.delta, u, O.indicator(o, u)[0])) <= RHS_TOL
+
+
 def test_amr_fp32_operator():
     N = 3
     f = _forest()

@@ -71,7 +71,14 @@ def test_autotune_restores_state():
         for r, u in enumerate(states):
             b.upload(r, u)
         tuned = b.autotune(steps=2)
+        table = b.tune_table()
         print("autotuned", tuned)
+        # every candidate was timed and the kept geometry is the fastest
+        assert len(table) == 9
+        assert {t["one_launch"] for t in table} == {True, False}
+        best = min(table, key=lambda t: t["ms_per_step"])
+        assert (best["one_launch"], best["order"], best["grid_per_cu"], best["xcd_map"]) == \
+            (tuned["one_launch"], tuned["order"], tuned["grid_per_cu"], tuned["xcd_map"])
         after = [b.download(r) for r in range(len(states))]
         b.run(3)
         ua = [b.download(r) for r in range(len(states))]

@@ -102,6 +102,7 @@ class Forest:
     extent: tuple = (1.0, 1.0, 1.0)
     origin: tuple = (0.0, 0.0, 0.0)
     _conn: tuple = field(default=None, repr=False, compare=False)
+    _fam: np.ndarray = field(default=None, repr=False, compare=False)
 
     # ---- construction ----------------------------------------------------
     @classmethod
@@ -132,7 +133,14 @@ class Forest:
         return self._replace(level, anchor)
 
     def family_starts(self) -> np.ndarray:
-        """Boolean mask: leaf i is child 0 of a complete family i..i+7."""
+        """Boolean mask: leaf i is child 0 of a complete family i..i+7 (a
+        copy of the forest's cached mask; leaves are never changed in
+        place)."""
+        if self._fam is None:
+            self._fam = self._family_starts()
+        return self._fam.copy()
+
+    def _family_starts(self) -> np.ndarray:
         n = self.nelem
         out = np.zeros(n, dtype=bool)
         if n < 8:

@@ -25,15 +25,14 @@ def face_neighbours_any(forest: amr.Forest, flag: np.ndarray) -> np.ndarray:
     """flag | (flag of any face neighbour), across conforming and 2:1 faces."""
     nbr, mortars = forest.connectivity()
     flag = np.asarray(flag, dtype=bool)
-    out = flag.copy()
+    n = flag.shape[0]
     conf = nbr >= 0
-    e, f = np.nonzero(conf)
-    np.logical_or.at(out, e, flag[nbr[e, f]])
+    out = flag | (conf & flag[np.where(conf, nbr, 0)]).any(axis=1)
     if len(mortars):
         big, small = mortars[:, 0], mortars[:, 2:]
-        np.logical_or.at(out, big, flag[small].any(axis=1))
-        for k in range(4):
-            np.logical_or.at(out, small[:, k], flag[big])
+        # a leaf may sit on several mortars: count the flagged ones
+        out |= np.bincount(big, weights=flag[small].any(axis=1), minlength=n) > 0
+        out |= np.bincount(small.ravel(), weights=np.repeat(flag[big], 4), minlength=n) > 0
     return out
 
 

@@ -153,6 +153,37 @@ def test_adapt_pruning_gives_the_same_forest():
     assert f.adapt(r, c, prune=True)[1] is None and f.adapt(r, c, prune=False)[1] is None
 
 
+def test_face_neighbour_flags_by_geometry():
+    """amr_run.face_neighbours_any marks a leaf when it or any leaf sharing
+    a face with it is flagged; checked against touching boxes found by
+    brute force (periodic), on a forest with many 2:1 faces."""
+    from nemo_amd import amr_run
+    rng = np.random.default_rng(4)
+    f = amr.Forest.uniform(2, minlevel=2, maxlevel=4)
+    for _ in range(2):
+        f = f.refine(rng.random(f.nelem) < 0.25).balance()
+    lo = f.anchor.astype(np.int64)
+    h = (amr.ROOT >> f.level).astype(np.int64)
+    n = f.nelem
+    touch = np.zeros((n, n), dtype=bool)
+    for d in range(3):
+        for s in (-1, 1):
+            # leaves whose face plane coincides and whose face rectangles overlap
+            plane_i = (lo[:, d] + (h if s > 0 else 0)) % amr.ROOT
+            plane_j = (lo[:, d] + (0 if s > 0 else h)) % amr.ROOT
+            same = plane_i[:, None] == plane_j[None, :]
+            ov = np.ones((n, n), dtype=bool)
+            for t in range(3):
+                if t != d:
+                    ov &= (lo[:, None, t] < lo[None, :, t] + h[None, :]) & (lo[None, :, t] < lo[:, None, t] + h[:, None])
+            touch |= same & ov
+    np.fill_diagonal(touch, False)
+    for _ in range(20):
+        flag = rng.random(n) < rng.random() * 0.3
+        want = flag | (touch & flag[None, :]).any(axis=1)
+        assert np.array_equal(amr_run.face_neighbours_any(f, flag), want)
+
+
 def test_adapt_refine_coarsen_inverse():
     f = _forest(lmax=5)
     flags = np.zeros(f.nelem, dtype=bool)

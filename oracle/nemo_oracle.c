@@ -1034,13 +1034,16 @@ static void rhs_element(const orc_cfg *c, const orc_ops *op, int64_t e, const in
                     /* Powell/GLM terms at the faces with the element's own
                      * node as inner state and the face average (the
                      * reference's faceavgs, kernel/dgsem/weakform/2d/
-                     * kernel_flux_mod.f90:156-192) */
+                     * kernel_flux_mod.f90:156-192); a mortar slot carries
+                     * its side's term already */
                     double nm[MAXV], np[MAXV];
-                    phys_nc(c, d, ul[0], unm, ul[0], nm);
-                    phys_nc(c, d, ul[M - 1], ul[M - 1], unp, np);
-                    for (int h = 0; h < V; ++h) {
-                        fm[h] = fm[h] + nm[h];
-                        fp[h] = fp[h] + np[h];
+                    if (em >= 0) {
+                        phys_nc(c, d, ul[0], unm, ul[0], nm);
+                        for (int h = 0; h < V; ++h) fm[h] = fm[h] + nm[h];
+                    }
+                    if (ep >= 0) {
+                        phys_nc(c, d, ul[M - 1], ul[M - 1], unp, np);
+                        for (int h = 0; h < V; ++h) fp[h] = fp[h] + np[h];
                     }
                 }
 
@@ -1191,10 +1194,15 @@ void orc_mortar_ops(int N, double *P, double *R) {
  * kernel_flux_mod.f90:100-121: big face values -> the four half faces
  * (P along both tangential axes), Riemann problem per small-face node with
  * the x+/y+/z+ side as u_L, small faces keep their fluxes, the big face
- * takes sum_k R (x) R of them. */
+ * takes sum_k R (x) R of them.  GLM-MHD adds the Powell/GLM term per side
+ * before the projection (kernel/mcfv/3d/kernel_flux_mod.f90:229-250, with
+ * the projection at :252-268): each side's inner state at the half-face
+ * node (the small element's own node, the big element's interpolated one)
+ * times the face average of the two. */
 void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux) {
     (void)nelem;
-    const int M = c->N + 1, M2 = M * M, M3 = M2 * M, V = 5;
+    const int M = c->N + 1, M2 = M * M, M3 = M2 * M, V = orc_nvars(c);
+    const int mhd = c->eq == ORC_EQ_MHD;
     double P[2 * MAXM * MAXM], R[2 * MAXM * MAXM];
     orc_mortar_ops(c->N, P, R);
 #pragma omp parallel for schedule(static)
@@ -1223,13 +1231,17 @@ void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double 
                         }
                         ub[h] = acc;
                     }
-                    if (big_left)
-                        phys_riemann(c, c->surface, d, ub, us, f);
-                    else
-                        phys_riemann(c, c->surface, d, us, ub, f);
+                    const double *uL = big_left ? ub : us, *uR = big_left ? us : ub;
+                    double ns[MAXV], nb[MAXV];
+                    phys_riemann(c, c->surface, d, uL, uR, f);
+                    for (int h = 0; h < V; ++h) ns[h] = nb[h] = 0.0;
+                    if (mhd) {
+                        phys_nc(c, d, us, uL, uR, ns);
+                        phys_nc(c, d, ub, uL, uR, nb);
+                    }
                     for (int h = 0; h < V; ++h) {
-                        fk[k][h][ta + M * tb] = f[h];
-                        flux[((5 * m + 1 + k) * V + h) * M2 + ta + M * tb] = f[h];
+                        fk[k][h][ta + M * tb] = f[h] + nb[h];
+                        flux[((5 * m + 1 + k) * V + h) * M2 + ta + M * tb] = f[h] + ns[h];
                     }
                 }
         }
@@ -1257,7 +1269,7 @@ void orc_rhs(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *
     const int V = orc_nvars(c);
     double *mflux = NULL;
     if (c->nmortar > 0) {
-        mflux = (double *)malloc(sizeof(double) * (size_t)(5 * c->nmortar) * 5 * (size_t)op.M2);
+        mflux = (double *)malloc(sizeof(double) * (size_t)(5 * c->nmortar) * (size_t)V * (size_t)op.M2);
         orc_mortar_fluxes(c, nelem, u, mflux);
     }
 #pragma omp parallel for schedule(static)

@@ -48,7 +48,7 @@ typedef struct {
     double mu0;         /* MHD only */
     double glm_ch;      /* MHD only: GLM cleaning speed */
     double glm_alpha;   /* MHD only: GLM damping */
-    int64_t nmortar;    /* Euler only: non-conforming (2:1) interfaces */
+    int64_t nmortar;    /* non-conforming (2:1) interfaces */
     const int32_t *mortars; /* [nmortar][6]: big element, its face, small_0..small_3;
                                faces on mortar m carry nbr = -(5m+1) (big) and
                                -(5m+2+k) (small_k); small_k covers half k&1 of the
@@ -95,7 +95,8 @@ int orc_phys_isvalid(const orc_cfg *c, const double *u);
 /* ---- 2:1 operators (rfaceMat/cfaceMat, refineMat/coarseMat) ------------- */
 /* P[(c*M+i)*M+p] = l_p((x_i + 2c - 1)/2), R[(c*M+p)*M+i] = w_i P[c][i][p] / (2 w_p) */
 void orc_mortar_ops(int N, double *P, double *R);
-/* Face fluxes of every mortar slot: flux[slot][var][M*M] (Euler). */
+/* Face fluxes of every mortar slot: flux[slot][var][M*M]; GLM-MHD slots
+ * include their side's non-conservative term. */
 void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux);
 
 /* ---- element operators -------------------------------------------------- */

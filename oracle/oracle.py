@@ -61,8 +61,6 @@ def make_cfg(N=3, eq=EQ_EULER, volume=VOL_WEAK, surface=SURF_RUSANOV,
 
 def with_mortars(cfg: Cfg, mortars) -> Cfg:
     """Copy of cfg that carries a mortar table (kept alive with the copy)."""
-    if cfg.eq != EQ_EULER and mortars is not None and len(mortars):
-        raise ValueError("non-conforming faces are restated for the Euler equations only")
     c = Cfg.from_buffer_copy(cfg)
     arr = np.ascontiguousarray(mortars if mortars is not None else np.zeros((0, 6)), dtype=np.int32)
     c._mortars = arr
@@ -286,10 +284,10 @@ def mortar_ops(N: int):
 
 
 def mortar_fluxes(cfg: Cfg, u: np.ndarray) -> np.ndarray:
-    """Face fluxes of every mortar slot, [5*nmortar, 5, M*M]."""
+    """Face fluxes of every mortar slot, [5*nmortar, nvars, M*M]."""
     u = np.ascontiguousarray(u, dtype=np.float64)
     M = cfg.N + 1
-    out = np.zeros((5 * cfg.nmortar, 5, M * M))
+    out = np.zeros((5 * cfg.nmortar, nvars(cfg), M * M))
     lib().orc_mortar_fluxes(ctypes.byref(cfg), u.shape[0], _ptr(u), _ptr(out))
     return out
 

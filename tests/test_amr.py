@@ -261,6 +261,31 @@ def test_oracle_mortar_exact_state(N, vol, surf):
     assert np.abs(r - ex)[inner].max() <= 1e-13 * scale
 
 
+@pytest.mark.parametrize("N", [2, 3])
+@pytest.mark.parametrize("vol,surf", [(O.VOL_CENTRAL, O.SURF_RUSANOV), (O.VOL_CENTRAL, O.SURF_HLLD)])
+def test_oracle_mhd_mortar_exact_state(N, vol, surf):
+    """GLM-MHD at rest in a uniform field (B constant, psi = 0) with the
+    pressure and density of the Euler case: the field adds a constant to the
+    momentum flux and nothing else, so the exact RHS is still (0, -grad p, 0,
+    0, 0); the 9-variable mortar path must reproduce it with the same halves
+    and orientation."""
+    f = _forest()
+    cfg = O.with_mortars(O.make_cfg(N=N, eq=O.EQ_MHD, gamma=1.4, mu0=1.0, volume=vol, surface=surf,
+                                    timedisc=O.TD_LSRK54, glm_ch=0.7), f.mortars)
+    ue, ex5 = _state_at_rest(f, N)
+    B = np.array([0.3, -0.5, 0.4])
+    u = np.zeros((f.nelem, 9, (N + 1) ** 3))
+    u[:, :5] = ue
+    u[:, 4] += 0.5 * (B @ B)
+    u[:, 5:8] = B[None, :, None]
+    ex = np.zeros_like(u)
+    ex[:, :5] = ex5
+    r = O.rhs(cfg, f.nbr, f.delta, u)
+    inner = np.all((f.lower > 1e-12) & (f.lower + f.delta < 1 - 1e-12), axis=1)
+    scale = np.abs(ex).max() * N / f.delta.min()
+    assert np.abs(r - ex)[inner].max() <= 1e-13 * scale
+
+
 @pytest.mark.parametrize("N", [2, 3, 4])
 def test_oracle_mortar_free_stream_and_conservation(N):
     f = _forest()
@@ -349,3 +374,174 @@ def test_oracle_remap_positivity_fallback():
         rho = ug[e, 0]
         p = 0.4 * (ug[e, 4] - 0.5 * (ug[e, 1:4] ** 2).sum(0) / rho)
         assert (rho > 0).all() and (p > 0).all()
+
+
+# ---- the same faces on a non-conforming and a conforming mesh -----------------------
+
+def _slab_meshes(level=2):
+    """(coarse, mixed, fine): uniform forests at `level` and `level + 1`, and
+    the coarse one with the slab of elements at x-index 1 refined, so that
+    every non-conforming face has its normal along x."""
+    coarse = amr.Forest.uniform(level, maxlevel=level + 1)
+    h = 1.0 / (1 << level)
+    cen = coarse.lower + 0.5 * coarse.delta
+    mixed = coarse.refine(np.abs(cen[:, 0] - 1.5 * h) < 0.25 * h).balance()
+    fine = amr.Forest.uniform(level + 1, maxlevel=level + 1)
+    return coarse, mixed, fine, h
+
+
+def _slab_state(f, N, h, eq, seed=3):
+    """A state that varies along x only, as a polynomial of degree N within
+    each coarse cell [i h, (i+1) h) and discontinuous between cells: every
+    element of every mesh holds it exactly, so the faces between elements of
+    one coarse cell are continuous and the faces between cells carry the same
+    Riemann problem on every mesh."""
+    rng = np.random.default_rng(seed)
+    M = N + 1
+    ncell = int(round(1.0 / h))
+    nv = 9 if eq == O.EQ_MHD else 5
+    # per cell and variable: polynomial coefficients in the local coordinate
+    coef = 0.1 * rng.uniform(-1, 1, (ncell, nv, N + 1))
+    base = np.array([1.2, 0.3, -0.2, 0.1, 6.0, 0.5, -0.4, 0.6, 0.1][:nv])
+    x, _ = O.lgl(N)
+    X = f.node_coords(x)
+    cell = np.floor((f.lower[:, 0] + 0.5 * f.delta[:, 0]) / h).astype(int)
+    xi = (X[:, 0] - cell[:, None] * h) / h
+    u = np.empty((f.nelem, nv, M ** 3))
+    for v in range(nv):
+        u[:, v] = base[v] + sum(coef[cell, v, k][:, None] * xi ** k for k in range(N + 1))
+    return u
+
+
+def _by_position(f):
+    return {tuple(np.round(lo * 1024).astype(int)) + (int(lv),): e
+            for e, (lo, lv) in enumerate(zip(f.lower, f.level))}
+
+
+def _slab_cfgs(N):
+    yield O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP)
+    yield O.make_cfg(N=N, volume=O.VOL_WEAK, surface=O.SURF_RUSANOV, indicator=O.IND_PP)
+    mhd = dict(N=N, eq=O.EQ_MHD, gamma=5.0 / 3.0, mu0=1.0, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+               glm_ch=0.8, glm_alpha=0.5)
+    yield O.make_cfg(volume=O.VOL_EC, surface=O.SURF_HLLD, **mhd)
+    yield O.make_cfg(volume=O.VOL_CENTRAL, surface=O.SURF_RUSANOV, **mhd)
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_oracle_mortar_matches_conforming_faces(N):
+    """With a state constant across x-normal faces (tangentially) the mortar
+    sequence has a closed form: interpolating the big face is exact, every
+    half face carries the conforming face's Riemann problem, and the
+    projection of a tangentially constant flux is that flux.  So on the slab
+    mesh the RHS of every big element equals its RHS on the coarse uniform
+    mesh and that of every small element its RHS on the fine uniform mesh, to
+    rounding, for any blending factor and, for GLM-MHD, with the
+    non-conservative term of each side (the small side's own node, the big
+    side's interpolated one).  A wrong side state in that term, a term added
+    twice or dropped, or a transposed projection fails it."""
+    coarse, mixed, fine, h = _slab_meshes()
+    assert len(mixed.mortars) and np.all(mixed.mortars[:, 1] // 2 == 0)
+    pc, pf = _by_position(coarse), _by_position(fine)
+    on_mortar = np.zeros(mixed.nelem, bool)
+    on_mortar[mixed.mortars[:, 0]] = True
+    on_mortar[mixed.mortars[:, 2:].ravel()] = True
+    for cfg in _slab_cfgs(N):
+        eq = cfg.eq
+        um, uc, uf = (_slab_state(f, N, h, eq) for f in (mixed, coarse, fine))
+        am = np.full(mixed.nelem, 0.3)
+        rm = O.rhs(O.with_mortars(cfg, mixed.mortars), mixed.nbr, mixed.delta, um, am)
+        rc = O.rhs(cfg, coarse.nbr, coarse.delta, uc, np.full(coarse.nelem, 0.3))
+        rf = O.rhs(cfg, fine.nbr, fine.delta, uf, np.full(fine.nelem, 0.3))
+        scale = max(np.abs(rc).max(), np.abs(rf).max())
+        assert scale > 1.0
+        worst = 0.0
+        for e, key in enumerate(_by_position(mixed)):
+            ref = rc[pc[key]] if key in pc else rf[pf[key]]
+            assert np.array_equal(um[e], (uc[pc[key]] if key in pc else uf[pf[key]]))
+            worst = max(worst, np.abs(rm[e] - ref).max())
+        assert worst <= 1e-12 * scale, (eq, cfg.volume, cfg.surface, worst / scale)
+        # the mortar faces carry a real jump: a conforming pairing would differ
+        assert np.abs(rm[on_mortar]).max() > 1e-3 * scale
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_oracle_mhd_mortar_free_stream_and_mass(N):
+    """GLM-MHD on the refined-around-a-point forest: a constant state with
+    psi = 0 is steady, and the mass of a rough state is conserved across the
+    non-conforming faces (the Powell/GLM terms have no mass component)."""
+    f = _forest()
+    M3 = (N + 1) ** 3
+    cfg = O.with_mortars(O.make_cfg(N=N, eq=O.EQ_MHD, gamma=5.0 / 3.0, mu0=1.0, volume=O.VOL_EC,
+                                    surface=O.SURF_HLLD, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+                                    glm_ch=0.8, glm_alpha=0.0), f.mortars)
+    u0 = np.array([1.2, 0.36, -0.24, 0.12, 3.2, 0.4, 0.7, -0.5, 0.0])
+    u = np.ascontiguousarray(np.broadcast_to(u0[None, :, None], (f.nelem, 9, M3)))
+    r = O.rhs(cfg, f.nbr, f.delta, u, np.full(f.nelem, 0.4))
+    assert np.abs(r).max() < 1e-11 * (N + 1) / f.delta.min()
+    rng = np.random.default_rng(5)
+    u = u * (1.0 + 0.1 * rng.uniform(-1, 1, u.shape))
+    r = O.rhs(cfg, f.nbr, f.delta, u, rng.uniform(0, 0.5, f.nelem))
+    _, w = O.lgl(N)
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    W = w3[None, :] * (np.prod(f.delta, axis=1) / 8.0)[:, None]
+    assert abs((W * r[:, 0]).sum()) <= 1e-13 * (W * np.abs(r[:, 0])).sum()
+
+
+# Plausible mistakes in the mortar sequence, planted in a copy of the
+# oracle: each must make the slab comparison above fail.  (Its state is
+# constant along the faces, so a wrong half or orientation goes unseen
+# there; the exactly representable state at rest above catches those.)
+_PLANTED_MORTAR = [
+    ("big side takes the small side's term", "phys_nc(c, d, ub, uL, uR, nb);", "phys_nc(c, d, us, uL, uR, nb);"),
+    ("small side's term dropped", "flux[((5 * m + 1 + k) * V + h) * M2 + ta + M * tb] = f[h] + ns[h];",
+     "flux[((5 * m + 1 + k) * V + h) * M2 + ta + M * tb] = f[h];"),
+    ("big side's term after the projection", "fk[k][h][ta + M * tb] = f[h] + nb[h];", "fk[k][h][ta + M * tb] = f[h];"),
+    ("term added again at the element", "if (em >= 0) {\n                        phys_nc(c, d, ul[0], unm",
+     "if (1) {\n                        if (em < 0) for (int h = 0; h < V; ++h) unm[h] = ul[0][h];\n"
+     "                        phys_nc(c, d, ul[0], unm"),
+    ("projection transposed", "r += R[(ka * M + p) * M + ta] * fk[k][h][ta + M * tb];",
+     "r += R[(ka * M + ta) * M + p] * fk[k][h][ta + M * tb];"),
+]
+
+
+@pytest.mark.parametrize("name,old,new", _PLANTED_MORTAR, ids=[p[0] for p in _PLANTED_MORTAR])
+def test_planted_mortar_mistakes_fail_the_slab_comparison(tmp_path, name, old, new):
+    import ctypes
+    import os
+    import shutil
+    import subprocess
+    src = open(os.path.join(os.path.dirname(O.__file__), "nemo_oracle.c")).read()
+    assert src.count(old) == 1, name
+    d = tmp_path / "mut"
+    d.mkdir()
+    (d / "nemo_oracle.c").write_text(src.replace(old, new))
+    shutil.copy(os.path.join(os.path.dirname(O.__file__), "nemo_oracle.h"), d / "nemo_oracle.h")
+    so = d / "libmut.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-fopenmp", "-std=c11", "-shared", "-o", str(so),
+                    str(d / "nemo_oracle.c"), "-lm"], check=True)
+    L = ctypes.CDLL(str(so))
+    P, PI = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+    L.orc_rhs.argtypes = [ctypes.POINTER(O.Cfg), ctypes.c_int64, PI, P, P, P, P]
+
+    def rhs(cfg, f, u, alpha):
+        nb = np.ascontiguousarray(f.nbr, dtype=np.int32)
+        de = np.ascontiguousarray(f.delta, dtype=np.float64)
+        u = np.ascontiguousarray(u)
+        a = np.ascontiguousarray(alpha)
+        out = np.zeros_like(u)
+        L.orc_rhs(ctypes.byref(cfg), u.shape[0], nb.ctypes.data_as(PI), de.ctypes.data_as(P), u.ctypes.data_as(P),
+                  a.ctypes.data_as(P), out.ctypes.data_as(P))
+        return out
+
+    N = 3
+    coarse, mixed, fine, h = _slab_meshes()
+    pc, pf = _by_position(coarse), _by_position(fine)
+    cfg = list(_slab_cfgs(N))[2]  # GLM-MHD, EC volume, HLLD faces
+    um, uc, uf = (_slab_state(f, N, h, cfg.eq) for f in (mixed, coarse, fine))
+    rm = rhs(O.with_mortars(cfg, mixed.mortars), mixed, um, np.full(mixed.nelem, 0.3))
+    rc = O.rhs(cfg, coarse.nbr, coarse.delta, uc, np.full(coarse.nelem, 0.3))
+    rf = O.rhs(cfg, fine.nbr, fine.delta, uf, np.full(fine.nelem, 0.3))
+    scale = max(np.abs(rc).max(), np.abs(rf).max())
+    worst = max(np.abs(rm[e] - (rc[pc[k]] if k in pc else rf[pf[k]])).max()
+                for e, k in enumerate(_by_position(mixed)))
+    assert worst > 1e-6 * scale, (name, worst / scale)

@@ -98,8 +98,9 @@ int nemo_gpu_upload_mesh(void *handle, int64_t nelem, const int32_t *nbr, const 
  * face's first tangential axis and half (k >> 1) along the second (axes in
  * increasing order).  Faces on mortar m carry nbr = -(5m+1) (big) and
  * -(5m+2+k) (small_k).  Their fluxes follow the reference's rfaceMat /
- * cfaceMat sequence (kernel/dgsem/weakform/2d/kernel_flux_mod.f90:58-147).
- * Euler only. */
+ * cfaceMat sequence (kernel/dgsem/weakform/2d/kernel_flux_mod.f90:58-147);
+ * for GLM-MHD with each side's Powell/GLM term added before the projection
+ * (kernel/mcfv/3d/kernel_flux_mod.f90:229-268). */
 int nemo_gpu_upload_mesh_amr(void *handle, int64_t nelem, const int32_t *nbr, const double *delta,
                              int64_t nmortar, const int32_t *mortars);
 
@@ -110,9 +111,9 @@ int nemo_gpu_upload_mesh_amr(void *handle, int64_t nelem, const int32_t *nbr, co
  * element, first new element) in element order, kind 0 = copy, 1 = refine
  * (one old into the 8 new children, child c = x + 2y + 4z), 2 = coarsen (8
  * old siblings into one).  A unit whose interpolated or projected state has
- * a nonpositive density or pressure takes the (conserved) element mean
- * instead; their number is returned in *fallbacks.  RK registers are
- * cleared: call between time steps.  Euler only. */
+ * a nonpositive density or pressure (GLM-MHD: gas pressure) takes the
+ * (conserved) element mean instead; their number is returned in *fallbacks.
+ * RK registers are cleared: call between time steps. */
 int nemo_gpu_remesh(void *handle, int64_t nelem, const int32_t *nbr, const double *delta, int64_t nmortar,
                     const int32_t *mortars, int64_t nunits, const int32_t *units, int64_t *fallbacks);
 

@@ -55,12 +55,16 @@ __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, 
     const long long ec = valid ? e : 0;
     const Real jac = Real(2.0 / a.delta[ec * 3 + D]);
     {
+        // A face on a non-conforming interface takes its flux, with this
+        // side's non-conservative term, from the mortar pass: the face
+        // computation then runs on the element's own face node and is
+        // discarded.
+        const long long em = a.nbr[ec * 6 + 2 * D];
+        const long long ep = a.nbr[ec * 6 + 2 * D + 1];
         MhdState<Real> Nm, Np;
         {
-            const long long em = a.nbr[ec * 6 + 2 * D];
-            const long long ep = a.nbr[ec * 6 + 2 * D + 1];
-            const double* um = a.u_in + em * M3 + nidx<D, M>(M - 1, ta, tb);
-            const double* up = a.u_in + ep * M3 + nidx<D, M>(0, ta, tb);
+            const double* um = a.u_in + (em < 0 ? ec * M3 + nidx<D, M>(0, ta, tb) : em * M3 + nidx<D, M>(M - 1, ta, tb));
+            const double* up = a.u_in + (ep < 0 ? ec * M3 + nidx<D, M>(M - 1, ta, tb) : ep * M3 + nidx<D, M>(0, ta, tb));
 #pragma unroll
             for (int h = 0; h < V; ++h) {
                 Nm.u[h] = Real(um[h * EM3]);
@@ -74,6 +78,16 @@ __device__ __forceinline__ void direction_pass_mhd(const StageArgs<M, Real>& a, 
         mhd_nc_add<D>(g, S[0], Real(0.5) * (Nm.u[BN] + S[0].u[BN]), Real(0.5) * (Nm.u[GLMP] + S[0].u[GLMP]), fm);
         mhd_nc_add<D>(g, S[M - 1], Real(0.5) * (S[M - 1].u[BN] + Np.u[BN]),
                       Real(0.5) * (S[M - 1].u[GLMP] + Np.u[GLMP]), fp);
+        if (em < 0) {
+            const double* mf = a.mortar_flux + size_t(-em - 1) * V * M2 + ta + M * tb;
+#pragma unroll
+            for (int h = 0; h < V; ++h) fm[h] = Real(mf[h * M2]);
+        }
+        if (ep < 0) {
+            const double* mf = a.mortar_flux + size_t(-ep - 1) * V * M2 + ta + M * tb;
+#pragma unroll
+            for (int h = 0; h < V; ++h) fp[h] = Real(mf[h * M2]);
+        }
     }
 
     Real c[M][V];

@@ -7,16 +7,20 @@
 // the big element's face values are interpolated to the nodes of each of the
 // four half-size faces (rfaceMat), the Riemann problem is solved at those
 // nodes, each small element takes its fluxes as they are and the big element
-// takes their projection (cfaceMat) summed over the four.  One workgroup per
-// mortar, one lane per small-face node; the face fluxes go to slots of a
-// small buffer that the stage kernel reads in place of its own Riemann solve
-// on those faces.
+// takes their projection (cfaceMat) summed over the four.  GLM-MHD adds the
+// Powell/GLM term of each side before the projection (kernel/mcfv/3d/
+// kernel_flux_mod.f90:229-268): the side's inner state at the half-face node
+// (the small element's own node, the big element's interpolated one) times
+// the face averages of B_n and psi.  One workgroup per mortar, one lane per
+// small-face node; the face fluxes go to slots of a small buffer that the
+// stage kernel reads in place of its own face flux on those faces.
 //
 // remap_kernel: kernel_amr_refine / kernel_amr_coarsen
 // (kernel/mcfv/3d/kernel_amr_mod.f90:12-70) with the positivity fallback of
 // setup_amr_replace_refine/coarsen (setups/euler/3d/sedov/setup_amr_mod.f90:
-// 76-130): one workgroup per replacement unit (copy, one parent into eight
-// children, eight children into one parent), one lane per node.
+// 76-130; for GLM-MHD with that equation set's density and gas pressure):
+// one workgroup per replacement unit (copy, one parent into eight children,
+// eight children into one parent), one lane per node.
 #pragma once
 
 #include "dg_euler.hpp"
@@ -40,8 +44,10 @@ struct MortarArgs {
     const double* __restrict__ u_in;     // [var][elem][node]
     double* __restrict__ flux;           // [slot][var][M*M]
     const int32_t* __restrict__ mortars; // [nmortar][6]: big, face, small_0..3
+    const double* __restrict__ glm;      // GLM-MHD: c_h of the current step in glm[0]
     long long nelem;
     Gas<Real> gas;
+    MhdGas<Real> mhd;
     Phys64 ph64;
     MortarOps<M> op;
 };
@@ -52,29 +58,56 @@ __device__ __forceinline__ int face_node(int d, int l, int ta, int tb) {
     return d == 0 ? nidx<0, M>(l, ta, tb) : (d == 1 ? nidx<1, M>(l, ta, tb) : nidx<2, M>(l, ta, tb));
 }
 
-template <int SURF, int D, typename Real>
-__device__ __forceinline__ void mortar_riemann_d(const Gas<Real>& g, const Phys64& P, const double* uL,
-                                                 const double* uR, Real* f) {
-    NodeState<Real> L, R;
+// Face flux of one half-face node for each side: fs for the small element
+// (uL or uR per big_left), fb for the big one; equal for Euler.
+template <int EQ, int SURF, int D, typename Real>
+__device__ __forceinline__ void mortar_riemann_d(const Gas<Real>& g, const MhdGas<Real>& mg, const Phys64& P,
+                                                 const double* uL, const double* uR, bool big_left, Real* fs,
+                                                 Real* fb) {
+    if constexpr (EQ == EQ_MHD) {
+        constexpr int BN = Mag<D, 0>::idx;
+        MhdState<Real> L, R;
 #pragma unroll
-    for (int h = 0; h < 5; ++h) {
-        L.u[h] = Real(uL[h]);
-        R.u[h] = Real(uR[h]);
+        for (int h = 0; h < 9; ++h) {
+            L.u[h] = Real(uL[h]);
+            R.u[h] = Real(uR[h]);
+        }
+        mhd_make_state<false>(mg, L);
+        mhd_make_state<false>(mg, R);
+        if constexpr (SURF == SURF_HLLD)
+            mhd_hlld<D>(mg, L, R, fs);
+        else
+            mhd_rusanov<D>(mg, L, R, fs);
+#pragma unroll
+        for (int h = 0; h < 9; ++h) fb[h] = fs[h];
+        const Real Bavg = Real(0.5) * (L.u[BN] + R.u[BN]);
+        const Real Gavg = Real(0.5) * (L.u[GLMP] + R.u[GLMP]);
+        mhd_nc_add<D>(mg, big_left ? R : L, Bavg, Gavg, fs);
+        mhd_nc_add<D>(mg, big_left ? L : R, Bavg, Gavg, fb);
+    } else {
+        NodeState<Real> L, R;
+#pragma unroll
+        for (int h = 0; h < 5; ++h) {
+            L.u[h] = Real(uL[h]);
+            R.u[h] = Real(uR[h]);
+        }
+        if constexpr (sizeof(Real) == 4) {
+            L.p64 = pressure(P.gas, uL);
+            R.p64 = pressure(P.gas, uR);
+        }
+        make_state<D, false>(g, L);
+        make_state<D, false>(g, R);
+        riemann<SURF, D>(g, L, R, fs);
+#pragma unroll
+        for (int h = 0; h < 5; ++h) fb[h] = fs[h];
     }
-    if constexpr (sizeof(Real) == 4) {
-        L.p64 = pressure(P.gas, uL);
-        R.p64 = pressure(P.gas, uR);
-    }
-    make_state<D, false>(g, L);
-    make_state<D, false>(g, R);
-    riemann<SURF, D>(g, L, R, f);
 }
 
-template <int M, int SURF, typename Real>
+template <int EQ, int M, int SURF, typename Real>
 __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M, Real> a) {
-    constexpr int M2 = M * M, M3 = M2 * M;
-    __shared__ double s_big[5][M2];
-    __shared__ double s_f[5][4 * M2];
+    constexpr int M2 = M * M, M3 = M2 * M, V = NVars<EQ>::V;
+    __shared__ double s_big[V][M2];
+    __shared__ double s_f[V][4 * M2];
     const int tid = threadIdx.x;
     const long long m = blockIdx.x;
     const int32_t* mt = a.mortars + m * 6;
@@ -83,10 +116,12 @@ __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M
     const int d = fb >> 1;
     const bool big_left = (fb & 1) != 0;  // big face is x+/y+/z+: big state is u_L
     const long long EM3 = a.nelem * (long long)M3;
+    MhdGas<Real> mg = a.mhd;
+    if constexpr (EQ == EQ_MHD) mg.ch = Real(a.glm[0]);
     if (tid < M2) {
         const int n = face_node<M>(d, big_left ? M - 1 : 0, tid % M, tid / M);
 #pragma unroll
-        for (int h = 0; h < 5; ++h) s_big[h][tid] = a.u_in[h * EM3 + big * M3 + n];
+        for (int h = 0; h < V; ++h) s_big[h][tid] = a.u_in[h * EM3 + big * M3 + n];
     }
     __syncthreads();
     if (tid < 4 * M2) {
@@ -94,9 +129,9 @@ __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M
         const int ta = t % M, tb = t / M, ka = k & 1, kb = k >> 1;
         const long long sm = mt[2 + k];
         const int n = face_node<M>(d, big_left ? 0 : M - 1, ta, tb);
-        double us[5], ub[5];
+        double us[V], ub[V];
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
+        for (int h = 0; h < V; ++h) {
             us[h] = a.u_in[h * EM3 + sm * M3 + n];
             double acc = 0.0;
 #pragma unroll
@@ -110,26 +145,26 @@ __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M
         }
         const double* uL = big_left ? ub : us;
         const double* uR = big_left ? us : ub;
-        Real f[5];
+        Real fsm[V], fbg[V];
         if (d == 0)
-            mortar_riemann_d<SURF, 0>(a.gas, a.ph64, uL, uR, f);
+            mortar_riemann_d<EQ, SURF, 0>(a.gas, mg, a.ph64, uL, uR, big_left, fsm, fbg);
         else if (d == 1)
-            mortar_riemann_d<SURF, 1>(a.gas, a.ph64, uL, uR, f);
+            mortar_riemann_d<EQ, SURF, 1>(a.gas, mg, a.ph64, uL, uR, big_left, fsm, fbg);
         else
-            mortar_riemann_d<SURF, 2>(a.gas, a.ph64, uL, uR, f);
-        double* out = a.flux + (size_t(5 * m + 1 + k) * 5) * M2 + t;
+            mortar_riemann_d<EQ, SURF, 2>(a.gas, mg, a.ph64, uL, uR, big_left, fsm, fbg);
+        double* out = a.flux + (size_t(5 * m + 1 + k) * V) * M2 + t;
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
-            out[h * M2] = double(f[h]);
-            s_f[h][tid] = double(f[h]);
+        for (int h = 0; h < V; ++h) {
+            out[h * M2] = double(fsm[h]);
+            s_f[h][tid] = double(fbg[h]);
         }
     }
     __syncthreads();
     if (tid < M2) {
         const int p = tid % M, q = tid / M;
-        double* out = a.flux + (size_t(5 * m) * 5) * M2 + tid;
+        double* out = a.flux + (size_t(5 * m) * V) * M2 + tid;
 #pragma unroll
-        for (int h = 0; h < 5; ++h) {
+        for (int h = 0; h < V; ++h) {
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -148,7 +183,7 @@ __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M
 }
 
 // ---------------------------------------------------------------------------
-// State transfer of one adaptation (Euler, FP64)
+// State transfer of one adaptation (FP64, Euler or GLM-MHD)
 // ---------------------------------------------------------------------------
 
 enum { UNIT_COPY = 0, UNIT_REFINE = 1, UNIT_COARSEN = 2 };
@@ -187,10 +222,14 @@ __device__ __forceinline__ double block_sum(double v, double* s_red) {
     return r;
 }
 
-template <int M>
+// One variable at a time passes through LDS (the eight children of a
+// coarsening unit at most), so the footprint does not grow with the number
+// of variables; the results of all variables stay in registers until the
+// unit's validity is known.
+template <int EQ, int M>
 __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a) {
-    constexpr int M3 = M * M * M, NT = RemapShape<M>::NT;
-    __shared__ double s_in[8][5][M3];
+    constexpr int M3 = M * M * M, NT = RemapShape<M>::NT, V = NVars<EQ>::V;
+    __shared__ double s_in[8][M3];
     __shared__ double s_red[NT];
     const int tid = threadIdx.x;
     const int32_t* ut = a.units + blockIdx.x * 3;
@@ -204,25 +243,21 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
     if (kind == UNIT_COPY) {
         if (act)
 #pragma unroll
-            for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + jn * M3 + tid] = a.u_old[h * OM3 + io * M3 + tid];
+            for (int h = 0; h < V; ++h) a.u_new[h * NM3 + jn * M3 + tid] = a.u_old[h * OM3 + io * M3 + tid];
         return;
     }
-    const int nin = kind == UNIT_REFINE ? 1 : 8;
-    for (int c = 0; c < nin; ++c)
-        if (act)
-#pragma unroll
-            for (int h = 0; h < 5; ++h) s_in[c][h][tid] = a.u_old[h * OM3 + (io + c) * M3 + tid];
-    __syncthreads();
 
     if (kind == UNIT_REFINE) {
         // child c = cx + 2 cy + 4 cz at new element jn + c
-        double out[8][5];
-        bool ok = true;
+        double out[8][V];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        for (int h = 0; h < V; ++h) {
+            __syncthreads();  // the previous variable's reads are done
+            if (act) s_in[0][tid] = a.u_old[h * OM3 + io * M3 + tid];
+            __syncthreads();
 #pragma unroll
-            for (int h = 0; h < 5; ++h) {
+            for (int c = 0; c < 8; ++c) {
+                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
                 double acc = 0.0;
                 if (act)
                     for (int kk = 0; kk < M; ++kk) {
@@ -231,34 +266,43 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
                             double si = 0.0;
 #pragma unroll
                             for (int ii = 0; ii < M; ++ii)
-                                si += a.op.P[cx][i][ii] * s_in[0][h][ii + M * jj + M * M * kk];
+                                si += a.op.P[cx][i][ii] * s_in[0][ii + M * jj + M * M * kk];
                             sj += a.op.P[cy][j][jj] * si;
                         }
                         acc += a.op.P[cz][k][kk] * sj;
                     }
                 out[c][h] = acc;
             }
-            if (act) ok = ok && isvalid64<EQ_EULER>(a.ph64, out[c]);
         }
+        bool ok = true;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (act) ok = ok && isvalid64<EQ>(a.ph64, out[c]);
         const bool bad = __syncthreads_or(!ok);
-        double mean[5];
+        double mean[V];
         if (bad) {
 #pragma unroll
-            for (int h = 0; h < 5; ++h) mean[h] = block_sum<NT>(act ? wn * s_in[0][h][tid] : 0.0, s_red);
+            for (int h = 0; h < V; ++h)
+                mean[h] = block_sum<NT>(act ? wn * a.u_old[h * OM3 + io * M3 + tid] : 0.0, s_red);
             if (tid == 0) atomicAdd(a.fallbacks, 1);
         }
         if (act)
 #pragma unroll
             for (int c = 0; c < 8; ++c)
 #pragma unroll
-                for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + (jn + c) * M3 + tid] = bad ? mean[h] : out[c][h];
+                for (int h = 0; h < V; ++h) a.u_new[h * NM3 + (jn + c) * M3 + tid] = bad ? mean[h] : out[c][h];
         return;
     }
 
     // UNIT_COARSEN: children io .. io+7 (child order) into new element jn
-    double out[5];
+    double out[V];
 #pragma unroll
-    for (int h = 0; h < 5; ++h) {
+    for (int h = 0; h < V; ++h) {
+        __syncthreads();
+        if (act)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s_in[c][tid] = a.u_old[h * OM3 + (io + c) * M3 + tid];
+        __syncthreads();
         double acc = 0.0;
         if (act)
             for (int c = 0; c < 8; ++c) {
@@ -269,7 +313,7 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
                         double si = 0.0;
 #pragma unroll
                         for (int ii = 0; ii < M; ++ii)
-                            si += a.op.R[cx][i][ii] * s_in[c][h][ii + M * jj + M * M * kk];
+                            si += a.op.R[cx][i][ii] * s_in[c][ii + M * jj + M * M * kk];
                         sj += a.op.R[cy][j][jj] * si;
                     }
                     acc += a.op.R[cz][k][kk] * sj;
@@ -277,16 +321,16 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
             }
         out[h] = acc;
     }
-    const bool ok = !act || isvalid64<EQ_EULER>(a.ph64, out);
+    const bool ok = !act || isvalid64<EQ>(a.ph64, out);
     const bool bad = __syncthreads_or(!ok);
     if (bad) {
 #pragma unroll
-        for (int h = 0; h < 5; ++h) out[h] = block_sum<NT>(act ? wn * out[h] : 0.0, s_red);
+        for (int h = 0; h < V; ++h) out[h] = block_sum<NT>(act ? wn * out[h] : 0.0, s_red);
         if (tid == 0) atomicAdd(a.fallbacks, 1);
     }
     if (act)
 #pragma unroll
-        for (int h = 0; h < 5; ++h) a.u_new[h * NM3 + jn * M3 + tid] = out[h];
+        for (int h = 0; h < V; ++h) a.u_new[h * NM3 + jn * M3 + tid] = out[h];
 }
 
 }  // namespace nemo

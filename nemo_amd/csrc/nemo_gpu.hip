@@ -205,8 +205,6 @@ int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double*
     if (!nbr || !delta || nelem <= 0 || nelem > INT32_MAX) return fail(NEMO_ERR_ARG, "invalid mesh arguments");
     if (nmortar < 0 || (nmortar > 0 && !mortars) || nmortar > INT32_MAX / 5)
         return fail(NEMO_ERR_ARG, "invalid mortar arguments");
-    if (nmortar > 0 && s.cfg.equations != NEMO_EQ_EULER)
-        return fail(NEMO_ERR_ARG, "non-conforming faces are built for the Euler equations");
     std::vector<int32_t> owner(size_t(5 * nmortar), -1);
     for (int64_t i = 0; i < 6 * nelem; ++i) {
         const int32_t n = nbr[i];
@@ -276,7 +274,7 @@ int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, 
         s.nmortar = nmortar;
         if (nmortar > 0) {
             HIP_TRY(hipMalloc(&s.mortars, size_t(nmortar) * 6 * sizeof(int32_t)));
-            HIP_TRY(hipMalloc(&s.mortar_flux, size_t(nmortar) * 25 * size_t(s.M * s.M) * sizeof(double)));
+            HIP_TRY(hipMalloc(&s.mortar_flux, size_t(nmortar) * 5 * size_t(s.V) * size_t(s.M * s.M) * sizeof(double)));
         }
     }
     if (!s.counter) HIP_TRY(hipMalloc(&s.counter, sizeof(int)));
@@ -319,7 +317,11 @@ void launch_remap(Solver& s, double* u_new, const int32_t* d_units, int64_t nuni
     mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
     for (int i = 0; i < M; ++i) a.w[i] = s.ops.w[i];
     a.fallbacks = s.counter;
-    hipLaunchKernelGGL((nemo::remap_kernel<M>), dim3(unsigned(nunits)), dim3(nemo::RemapShape<M>::NT), 0, s.stream, a);
+    const dim3 grid{unsigned(nunits)}, block{unsigned(nemo::RemapShape<M>::NT)};
+    if (s.cfg.equations == NEMO_EQ_GLM_MHD)
+        hipLaunchKernelGGL((nemo::remap_kernel<nemo::EQ_MHD, M>), grid, block, 0, s.stream, a);
+    else
+        hipLaunchKernelGGL((nemo::remap_kernel<nemo::EQ_EULER, M>), grid, block, 0, s.stream, a);
 }
 
 }  // namespace
@@ -455,7 +457,6 @@ int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const doubl
                     const int32_t* mortars, int64_t nunits, const int32_t* units, int64_t* fallbacks) {
     Solver* s = static_cast<Solver*>(handle);
     if (int rc = check_ready(s)) return rc;
-    if (s->cfg.equations != NEMO_EQ_EULER) return fail(NEMO_ERR_ARG, "mesh adaptation is built for the Euler equations");
     if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
     if (nunits <= 0 || !units) return fail(NEMO_ERR_ARG, "invalid replacement units");
     // the units must pair every old element with every new one, in order

@@ -71,15 +71,28 @@ void launch_mortar(Solver& s) {
     a.u_in = s.u[s.cur];
     a.flux = s.mortar_flux;
     a.mortars = s.mortars;
+    a.glm = s.dt_dev + 3;
     a.nelem = s.nelem;
     a.gas = make_gas<Real>(s.cfg.gamma);
+    a.mhd = make_mhd<Real>(s.cfg.gamma, s.cfg.mu0);
     a.ph64 = make_phys64(s.cfg);
     mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
     const dim3 grid(unsigned(s.nmortar)), block(nemo::MortarShape<M>::NT);
-    if (s.cfg.surface == NEMO_SURF_HLLC)
-        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_HLLC, Real>), grid, block, 0, s.stream, a);
-    else
-        hipLaunchKernelGGL((nemo::mortar_kernel<M, nemo::SURF_RUSANOV, Real>), grid, block, 0, s.stream, a);
+    if (s.cfg.equations == NEMO_EQ_GLM_MHD) {
+        // GLM-MHD stages exist for n_poly = 3 only (the others fail after this)
+        if constexpr (M == 4) {
+            if (s.cfg.surface == NEMO_SURF_HLLD)
+                hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_HLLD, Real>), grid, block, 0,
+                                   s.stream, a);
+            else
+                hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_RUSANOV, Real>), grid, block, 0,
+                                   s.stream, a);
+        }
+    } else if (s.cfg.surface == NEMO_SURF_HLLC) {
+        hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_EULER, M, nemo::SURF_HLLC, Real>), grid, block, 0, s.stream, a);
+    } else {
+        hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_EULER, M, nemo::SURF_RUSANOV, Real>), grid, block, 0, s.stream, a);
+    }
 }
 
 template <int M, typename Real>

@@ -302,21 +302,39 @@ def _valid_euler(gamma: float, v: np.ndarray) -> bool:
     return bool(np.all(rho > 0.0) and np.all(p > 0.0))
 
 
-def remap(N: int, gamma: float, u_old: np.ndarray, units: np.ndarray, nnew: int):
+def _valid_mhd(gamma: float, mu0: float, v: np.ndarray) -> bool:
+    """rho > 0 and gas pressure > 0 at every node of v [..., 9, M^3]
+    (equations/mhd/equations_mod.f90:223-234: B^2 and psi^2 over 2 mu0 taken
+    out of the energy)."""
+    rho = v[..., 0, :]
+    mag = v[..., 5, :] ** 2 + v[..., 6, :] ** 2 + v[..., 7, :] ** 2 + v[..., 8, :] ** 2
+    p = (gamma - 1.0) * (v[..., 4, :] - 0.5 / rho * (v[..., 1, :] ** 2 + v[..., 2, :] ** 2 + v[..., 3, :] ** 2)
+                         - 0.5 / mu0 * mag)
+    return bool(np.all(rho > 0.0) and np.all(p > 0.0))
+
+
+def remap(N: int, gamma: float, u_old: np.ndarray, units: np.ndarray, nnew: int, mu0: Optional[float] = None):
     """State transfer of one adaptation: kernel_amr_refine / kernel_amr_coarsen
     (kernel/mcfv/3d/kernel_amr_mod.f90:12-70) as tensor products of the 2:1
     operators per direction, child c = x + 2y + 4z; a unit whose result has a
     nonpositive density or pressure takes the element mean instead (the
     positivity fallback of setup_amr_replace_refine/coarsen,
     setups/euler/3d/sedov/setup_amr_mod.f90:76-130, reduced to its last,
-    first-order step).  Returns (u_new [nnew, 5, M^3], number of fallbacks)."""
+    first-order step).  A state of 9 variables is GLM-MHD (gas pressure with
+    magnetic constant mu0).  Returns (u_new [nnew, V, M^3], number of
+    fallbacks)."""
     M = N + 1
+    V = u_old.shape[1]
+    if V == 9:
+        valid = lambda v: _valid_mhd(gamma, 1.0 if mu0 is None else mu0, v)
+    else:
+        valid = lambda v: _valid_euler(gamma, v)
     P, R = mortar_ops(N)
     _, w = lgl(N)
     w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1) / 8.0  # (k, j, i) order
     nold = u_old.shape[0]
-    uo = u_old.reshape(nold, 5, M, M, M)       # node axes (k, j, i)
-    out = np.zeros((nnew, 5, M, M, M))
+    uo = u_old.reshape(nold, V, M, M, M)       # node axes (k, j, i)
+    out = np.zeros((nnew, V, M, M, M))
     fallbacks = 0
     for kind, io, jn in units:
         if kind == UNIT_COPY:
@@ -324,20 +342,20 @@ def remap(N: int, gamma: float, u_old: np.ndarray, units: np.ndarray, nnew: int)
         elif kind == UNIT_REFINE:
             ch = np.stack([np.einsum("ka,jb,ic,vabc->vkji", P[c >> 2], P[(c >> 1) & 1], P[c & 1], uo[io])
                            for c in range(8)])
-            if not _valid_euler(gamma, ch.reshape(8, 5, -1)):
-                mean = uo[io].reshape(5, -1) @ w3
+            if not valid(ch.reshape(8, V, -1)):
+                mean = uo[io].reshape(V, -1) @ w3
                 ch[:] = mean[None, :, None, None, None]
                 fallbacks += 1
             out[jn:jn + 8] = ch
         else:
             par = sum(np.einsum("ka,jb,ic,vabc->vkji", R[c >> 2], R[(c >> 1) & 1], R[c & 1], uo[io + c])
                       for c in range(8))
-            if not _valid_euler(gamma, par.reshape(5, -1)):
-                mean = par.reshape(5, -1) @ w3
+            if not valid(par.reshape(V, -1)):
+                mean = par.reshape(V, -1) @ w3
                 par = np.broadcast_to(mean[:, None, None, None], par.shape).copy()
                 fallbacks += 1
             out[jn] = par
-    return out.reshape(nnew, 5, M ** 3), fallbacks
+    return out.reshape(nnew, V, M ** 3), fallbacks
 
 
 def calc_timestep(cfg: Cfg, delta: np.ndarray, u: np.ndarray):

@@ -376,6 +376,45 @@ def test_oracle_remap_positivity_fallback():
         assert (rho > 0).all() and (p > 0).all()
 
 
+def test_oracle_remap_mhd_exact_and_magnetic_fallback():
+    """GLM-MHD states (9 variables): refinement is exact for polynomials of
+    degree <= N in every variable, and the fallback tests the gas pressure
+    with the magnetic (and psi) energy taken out: a unit whose interpolated
+    gas pressure turns negative under a strong uniform field falls back,
+    although its total energy, read as an Euler state, stays positive."""
+    N = 3
+    M = N + 1
+    f = amr.Forest.uniform(1, maxlevel=2)
+    x, w = O.lgl(N)
+    X = f.node_coords(x)
+    xx, yy, zz = X[:, 0], X[:, 1], X[:, 2]
+    u = np.zeros((f.nelem, 9, M ** 3))
+    u[:, :5] = _poly_state(f, N)
+    u[:, 4] += 5.0
+    u[:, 5], u[:, 6], u[:, 7], u[:, 8] = 0.5 + 0.1 * xx * yy, -0.3 * zz ** 3, 0.2, 0.01 * yy
+    g, units = f.adapt(np.ones(f.nelem, bool), np.zeros(f.nelem, bool))
+    ug, nfb = O.remap(N, 5.0 / 3.0, u, units, g.nelem, mu0=1.0)
+    assert nfb == 0
+    Xg = g.node_coords(x)
+    assert np.abs(ug[:, :5] - (_poly_state(g, N) + np.array([0, 0, 0, 0, 5.0])[None, :, None])).max() < 1e-13
+    assert np.abs(ug[:, 5] - (0.5 + 0.1 * Xg[:, 0] * Xg[:, 1])).max() < 1e-13
+    assert np.abs(ug[:, 6] + 0.3 * Xg[:, 2] ** 3).max() < 1e-13
+    # gas pressure (1e-3, 1e-8, 1e-8, 1e-3) along x under B = (1, 0, 0): the
+    # interpolant is negative at a child node, the total energy is not
+    i = np.arange(M ** 3) % M
+    u = np.zeros((f.nelem, 9, M ** 3))
+    u[:, 0] = 1.0
+    u[:, 5] = 1.0
+    u[:, 4] = np.where((i == 0) | (i == M - 1), 1e-3, 1e-8)[None, :] * 1.5 + 0.5
+    ug, nfb = O.remap(N, 5.0 / 3.0, u, units, g.nelem, mu0=1.0)
+    assert nfb == f.nelem
+    _, nfb_euler = O.remap(N, 5.0 / 3.0, u[:, :5], units, g.nelem)
+    assert nfb_euler == 0
+    w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+    assert np.allclose(np.einsum("evn,n,e->v", ug, w3, np.prod(g.delta, 1)),
+                       np.einsum("evn,n,e->v", u, w3, np.prod(f.delta, 1)), rtol=1e-14, atol=1e-18)
+
+
 # ---- the same faces on a non-conforming and a conforming mesh -----------------------
 
 def _slab_meshes(level=2):

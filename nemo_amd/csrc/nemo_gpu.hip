@@ -398,8 +398,7 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         hipMalloc(&s->mlp_v, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
         hipMalloc(&s->mlp_stats, 3 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
-        (void)hipStreamDestroy(s->stream);
-        delete s;
+        nemo_gpu_destroy(s);  // frees whichever of them were allocated
         return fail(NEMO_ERR_HIP, "hipMalloc failed");
     }
     // this device's copy of the allocation-free analysis scratch

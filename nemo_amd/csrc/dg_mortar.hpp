@@ -222,14 +222,21 @@ __device__ __forceinline__ double block_sum(double v, double* s_red) {
     return r;
 }
 
-// One variable at a time passes through LDS (the eight children of a
-// coarsening unit at most), so the footprint does not grow with the number
-// of variables; the results of all variables stay in registers until the
+// The variables pass through LDS in batches of VB (all five for Euler, as
+// one batch: every load of a unit is issued before the first use; three for
+// GLM-MHD, which keeps the eight children of a coarsening unit within 24 KB
+// at N = 4); the results of all variables stay in registers until the
 // unit's validity is known.
+template <int EQ>
+struct RemapBatch {
+    static constexpr int VB = EQ == EQ_MHD ? 3 : 5;
+};
+
 template <int EQ, int M>
 __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a) {
-    constexpr int M3 = M * M * M, NT = RemapShape<M>::NT, V = NVars<EQ>::V;
-    __shared__ double s_in[8][M3];
+    constexpr int M3 = M * M * M, NT = RemapShape<M>::NT, V = NVars<EQ>::V, VB = RemapBatch<EQ>::VB;
+    static_assert(V % VB == 0, "whole batches");
+    __shared__ double s_in[8][VB][M3];
     __shared__ double s_red[NT];
     const int tid = threadIdx.x;
     const int32_t* ut = a.units + blockIdx.x * 3;
@@ -251,27 +258,32 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
         // child c = cx + 2 cy + 4 cz at new element jn + c
         double out[8][V];
 #pragma unroll
-        for (int h = 0; h < V; ++h) {
-            __syncthreads();  // the previous variable's reads are done
-            if (act) s_in[0][tid] = a.u_old[h * OM3 + io * M3 + tid];
+        for (int h0 = 0; h0 < V; h0 += VB) {
+            __syncthreads();  // the previous batch's reads are done
+            if (act)
+#pragma unroll
+                for (int hb = 0; hb < VB; ++hb) s_in[0][hb][tid] = a.u_old[(h0 + hb) * OM3 + io * M3 + tid];
             __syncthreads();
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-                double acc = 0.0;
-                if (act)
-                    for (int kk = 0; kk < M; ++kk) {
-                        double sj = 0.0;
-                        for (int jj = 0; jj < M; ++jj) {
-                            double si = 0.0;
 #pragma unroll
-                            for (int ii = 0; ii < M; ++ii)
-                                si += a.op.P[cx][i][ii] * s_in[0][ii + M * jj + M * M * kk];
-                            sj += a.op.P[cy][j][jj] * si;
+                for (int hb = 0; hb < VB; ++hb) {
+                    double acc = 0.0;
+                    if (act)
+                        for (int kk = 0; kk < M; ++kk) {
+                            double sj = 0.0;
+                            for (int jj = 0; jj < M; ++jj) {
+                                double si = 0.0;
+#pragma unroll
+                                for (int ii = 0; ii < M; ++ii)
+                                    si += a.op.P[cx][i][ii] * s_in[0][hb][ii + M * jj + M * M * kk];
+                                sj += a.op.P[cy][j][jj] * si;
+                            }
+                            acc += a.op.P[cz][k][kk] * sj;
                         }
-                        acc += a.op.P[cz][k][kk] * sj;
-                    }
-                out[c][h] = acc;
+                    out[c][h0 + hb] = acc;
+                }
             }
         }
         bool ok = true;
@@ -297,29 +309,34 @@ __global__ void __launch_bounds__(RemapShape<M>::NT) remap_kernel(RemapArgs<M> a
     // UNIT_COARSEN: children io .. io+7 (child order) into new element jn
     double out[V];
 #pragma unroll
-    for (int h = 0; h < V; ++h) {
+    for (int h0 = 0; h0 < V; h0 += VB) {
         __syncthreads();
         if (act)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) s_in[c][tid] = a.u_old[h * OM3 + (io + c) * M3 + tid];
-        __syncthreads();
-        double acc = 0.0;
-        if (act)
-            for (int c = 0; c < 8; ++c) {
-                const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-                for (int kk = 0; kk < M; ++kk) {
-                    double sj = 0.0;
-                    for (int jj = 0; jj < M; ++jj) {
-                        double si = 0.0;
+            for (int c = 0; c < 8; ++c)
 #pragma unroll
-                        for (int ii = 0; ii < M; ++ii)
-                            si += a.op.R[cx][i][ii] * s_in[c][ii + M * jj + M * M * kk];
-                        sj += a.op.R[cy][j][jj] * si;
+                for (int hb = 0; hb < VB; ++hb) s_in[c][hb][tid] = a.u_old[(h0 + hb) * OM3 + (io + c) * M3 + tid];
+        __syncthreads();
+#pragma unroll
+        for (int hb = 0; hb < VB; ++hb) {
+            double acc = 0.0;
+            if (act)
+                for (int c = 0; c < 8; ++c) {
+                    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                    for (int kk = 0; kk < M; ++kk) {
+                        double sj = 0.0;
+                        for (int jj = 0; jj < M; ++jj) {
+                            double si = 0.0;
+#pragma unroll
+                            for (int ii = 0; ii < M; ++ii)
+                                si += a.op.R[cx][i][ii] * s_in[c][hb][ii + M * jj + M * M * kk];
+                            sj += a.op.R[cy][j][jj] * si;
+                        }
+                        acc += a.op.R[cz][k][kk] * sj;
                     }
-                    acc += a.op.R[cz][k][kk] * sj;
                 }
-            }
-        out[h] = acc;
+            out[h0 + hb] = acc;
+        }
     }
     const bool ok = !act || isvalid64<EQ>(a.ph64, out);
     const bool bad = __syncthreads_or(!ok);

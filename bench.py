@@ -357,8 +357,8 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE in counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU (asked for --gpus {args.gpus}); "
+              "sharding over several GPUs is a design only", file=sys.stderr)
         sys.exit(2)
 
     from nemo_amd import _lib

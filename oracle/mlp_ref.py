@@ -2,7 +2,7 @@
 backward), the checker of the GPU training and inference kernels.
 
 TEST INFRASTRUCTURE ONLY: only tests/ and ``__graft_entry__.smoke()`` may
-import it.  The network is this build's own (the reference has no learned
+import it.  The network is the library's own (the reference has no learned
 indicator), so its checks are mathematical: tests/test_mlp_ref.py pins the
 gradient against central finite differences.
 

@@ -85,12 +85,12 @@ def test_amr_rhs_parity(combo):
     assert err <= RHS_TOL
 
 
-@pytest.mark.parametrize("N", [2, 3])
-def test_amr_run_parity_and_conservation(N):
+@pytest.mark.parametrize("N,td", [(2, _lib.TD_LSRK54), (3, _lib.TD_LSRK54), (3, _lib.TD_SSPRK54)])
+def test_amr_run_parity_and_conservation(N, td):
     f = _forest()
     u0 = _state(f, N)
-    g, o = _cfgs(f, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, repair=1)
-    print(f"amr run parity N={N}", flush=True)
+    g, o = _cfgs(f, N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, td=td, repair=1)
+    print(f"amr run parity N={N} timedisc={td}", flush=True)
     with solver.Solver(g) as s:
         s.upload_mesh(f)
         s.set_state(u0)
@@ -108,7 +108,15 @@ def test_amr_run_parity_and_conservation(N):
     vol = np.prod(f.delta, 1) / 8.0
     m0 = np.einsum("evn,n,e->v", u0, w3, vol)
     m1 = np.einsum("evn,n,e->v", u, w3, vol)
-    assert np.abs(m1[[0, 4]] - m0[[0, 4]]).max() <= 1e-13 * np.abs(m0[[0, 4]]).max()
+    assert np.abs(m1[[0, 4]] - m0[[0, 4]]).max() <= _drift_bound(td) * np.abs(m0[[0, 4]]).max()
+
+
+def _drift_bound(td):
+    """Bound on the relative change of the conserved totals over 20 steps:
+    LSRK adds increments to the state (the oracle drifts by 1e-15), SSPRK's
+    Shu-Osher stages form convex combinations of whole states, rounding
+    each node's state every stage (the oracle drifts by 2e-13)."""
+    return 1e-13 if td == _lib.TD_LSRK54 else 1e-12
 
 
 @pytest.mark.parametrize("N", [2, 3, 4])
@@ -405,11 +413,11 @@ def _mhd_state(f, N=3):
     return np.ascontiguousarray(setups.mhd_prim2cons(rho, v, p, B, psi, GAMMA_MHD))
 
 
-def _mhd_cfgs(f, vol, surf, ind, repair=0, precision=0, glm=(0.0, 0.0)):
+def _mhd_cfgs(f, vol, surf, ind, repair=0, precision=0, glm=(0.0, 0.0), td=_lib.TD_LSRK54):
     g = solver.make_config(N=3, equations=_lib.EQ_GLM_MHD, volume=vol, surface=surf, indicator=ind,
-                           timedisc=_lib.TD_LSRK54, repair=repair, gamma=GAMMA_MHD, precision=precision, mu0=1.0)
+                           timedisc=td, repair=repair, gamma=GAMMA_MHD, precision=precision, mu0=1.0)
     o = O.with_mortars(O.make_cfg(N=3, eq=O.EQ_MHD, volume=vol, surface=surf, indicator=ind,
-                                  timedisc=O.TD_LSRK54, repair=repair, gamma=GAMMA_MHD, mu0=1.0, glm_ch=glm[0],
+                                  timedisc=td, repair=repair, gamma=GAMMA_MHD, mu0=1.0, glm_ch=glm[0],
                                   glm_alpha=glm[1]), f.mortars)
     return g, o
 
@@ -447,15 +455,16 @@ def test_mhd_amr_rhs_parity(combo):
     assert max(err) <= RHS_TOL
 
 
-def test_mhd_amr_run_parity():
-    """20 LSRK steps of GLM-MHD (EC + HLLD, Persson-Peraire blending, repair
-    on, the cleaning speed and damping from each step's CFL reduction) on the
-    2:1 forest against the oracle's run: <= 1e-10 per variable; the mass is
-    conserved across the non-conforming faces."""
+@pytest.mark.parametrize("surf,td", [(_lib.SURF_HLLD, _lib.TD_LSRK54), (_lib.SURF_RUSANOV, _lib.TD_SSPRK54)])
+def test_mhd_amr_run_parity(surf, td):
+    """20 RK steps of GLM-MHD (EC volume flux, Persson-Peraire blending,
+    repair on, the cleaning speed and damping from each step's CFL
+    reduction) on the 2:1 forest against the oracle's run: <= 1e-10 per
+    variable; the mass is conserved across the non-conforming faces."""
     f = _forest()
     u0 = _mhd_state(f)
-    g, o = _mhd_cfgs(f, _lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP, repair=1)
-    print("mhd amr run parity", flush=True)
+    g, o = _mhd_cfgs(f, _lib.VOL_EC, surf, _lib.IND_PP, repair=1, td=td)
+    print(f"mhd amr run parity surf={surf} timedisc={td}", flush=True)
     with solver.Solver(g) as s:
         s.upload_mesh(f)
         s.set_state(u0)
@@ -472,7 +481,7 @@ def test_mhd_amr_run_parity():
     w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
     vol = np.prod(f.delta, 1) / 8.0
     m0, m1 = (np.einsum("en,n,e->", x[:, 0], w3, vol) for x in (u0, u))
-    assert abs(m1 - m0) <= 1e-13 * m0
+    assert abs(m1 - m0) <= _drift_bound(td) * m0
 
 
 def test_mhd_amr_fp32_operator():

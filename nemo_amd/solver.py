@@ -132,7 +132,8 @@ class Solver:
 
     def set_state(self, u: np.ndarray) -> None:
         u = np.ascontiguousarray(u, dtype=np.float64)
-        assert u.shape == (self.nelem, self.nvars, self.M3), u.shape
+        if u.shape != (self.nelem, self.nvars, self.M3):
+            raise ValueError(f"state of shape {u.shape}, expected {(self.nelem, self.nvars, self.M3)}")
         self._call("nemo_gpu_upload_state", self._h, dptr(u))
 
     def get_state(self) -> np.ndarray:
@@ -191,7 +192,8 @@ class Solver:
 
     def mlp_set_params(self, params: np.ndarray) -> None:
         p = np.ascontiguousarray(params, dtype=np.float32)
-        assert p.size == _lib.MLP_NPARAMS
+        if p.size != _lib.MLP_NPARAMS:
+            raise ValueError(f"{p.size} network parameters, expected {_lib.MLP_NPARAMS}")
         self._call("nemo_gpu_mlp_set_params", self._h, fptr(p))
 
     def mlp_get_params(self) -> np.ndarray:
@@ -365,7 +367,8 @@ class Batch:
         m = self.meshes[run]
         M3 = (self.n_polys[run] + 1) ** 3
         u = np.ascontiguousarray(u, dtype=np.float64)
-        assert u.shape == (m.nelem, 5, M3), u.shape
+        if u.shape != (m.nelem, 5, M3):
+            raise ValueError(f"state of run {run} has shape {u.shape}, expected {(m.nelem, 5, M3)}")
         nbr = np.ascontiguousarray(m.nbr, dtype=np.int32)
         delta = np.ascontiguousarray(m.delta, dtype=np.float64)
         self._call("nemo_gpu_batch_upload", self._h, ctypes.c_int32(run), iptr(nbr), dptr(delta), dptr(u))

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the MI355X spatial operator + RK stage update.
 
-Workload (the configuration the metric is quoted on): 3-D Euler on the
+Workload of the headline throughput: 3-D Euler on the
 periodic unit cube, 64^3 elements, N=3 (4^3 LGL nodes, 16.8M DOF per
 variable), Kelvin-Helmholtz data (seed 1234), HLLC surface flux,
 entropy-conservative (Chandrashekar) flux differencing, DG/FV subcell

@@ -15,9 +15,12 @@
 !!     heartbeat          -> perf index (driver_mod.f90:140-149)
 !!
 !! Usage: nemo_gpu_driver SETUP N_ELEM N_POLY NSTEPS PREFIX
-!!   SETUP  = vortex (config 1 scheme: weak form, Rusanov, SSPRK(5,4))
-!!          | kh     (config 2 scheme: EC flux differencing, HLLC, PP
+!!   SETUP  = vortex (isentropic vortex: weak form, Rusanov, SSPRK(5,4))
+!!          | kh     (Kelvin-Helmholtz: EC flux differencing, HLLC, PP
 !!                    blending, LSRK 5-4, positivity repair)
+!!          | ot     (Orszag-Tang, GLM-MHD: EC flux differencing, HLLD,
+!!                    PP blending, LSRK 5-4, positivity repair; the GLM
+!!                    speed and damping follow each step's CFL reduction)
 !! Writes PREFIX.init.bin, PREFIX.final.bin (nemo_setup_mod::write_state)
 !! and PREFIX.dt.txt (one time step per line).
 !! ========================================================================== !!
@@ -56,7 +59,7 @@ type(nemo_gpu_config) :: cfg
 type(c_ptr) :: h
 real(dp), allocatable :: state(:,:,:,:,:), xi(:), wi(:), dmat(:), vinv(:), dts(:)
 character(len=1024) :: setup, prefix, arg
-integer :: n, n_poly, nsteps, nn, unit, it
+integer :: n, n_poly, nsteps, nn, nvars, unit, it
 integer(c_int) :: rc
 integer(c_int64_t) :: t0, t1, rate
 real(dp) :: perfidx
@@ -86,6 +89,14 @@ case ('vortex')
 case ('kh')
     cfg%volume = NEMO_VOL_EC
     cfg%surface = NEMO_SURF_HLLC
+    cfg%indicator = NEMO_IND_PP
+    cfg%timedisc = NEMO_TD_LSRK54
+    cfg%repair = 1
+case ('ot')
+    cfg%equations = NEMO_EQ_GLM_MHD
+    cfg%gamma = 5.0_dp/3.0_dp
+    cfg%volume = NEMO_VOL_EC
+    cfg%surface = NEMO_SURF_HLLD
     cfg%indicator = NEMO_IND_PP
     cfg%timedisc = NEMO_TD_LSRK54
     cfg%repair = 1
@@ -141,12 +152,16 @@ subroutine hook_init()
     call check(nemo_gpu_create(cfg, h), 'nemo_gpu_create')
     call mesh_uniform(n, mesh)
     call check(nemo_gpu_upload_mesh(h, mesh%nelem, mesh%nbr, mesh%delta), 'nemo_gpu_upload_mesh')
-    allocate(state(nn, nn, nn, 5, mesh%nelem))
-    if (trim(setup) == 'vortex') then
+    nvars = merge(9, 5, cfg%equations == NEMO_EQ_GLM_MHD)
+    allocate(state(nn, nn, nn, nvars, mesh%nelem))
+    select case (trim(setup))
+    case ('vortex')
         call fill_vortex(mesh, nn, xi, cfg%gamma, state)
-    else
+    case ('kh')
         call fill_kh(mesh, nn, xi, cfg%gamma, 0.01_dp, state)
-    end if
+    case default
+        call fill_orszag_tang(mesh, nn, xi, cfg%gamma, 1.0e-3_dp, state)
+    end select
     call check(nemo_gpu_upload_state(h, state), 'nemo_gpu_upload_state')
 end subroutine
 

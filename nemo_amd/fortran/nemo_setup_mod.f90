@@ -149,6 +149,47 @@ subroutine fill_kh(mesh, nn, xi, kappa, noise, state)
     end do
 end subroutine
 
+!! prim2cons of GLM-MHD: equations/mhd/equations_mod.f90:294-310 (mu0 = 1;
+!! the energy holds B^2/2 and psi^2/2).
+pure subroutine prim2cons_mhd(kappa, rho, v, p, b, psi, u)
+    real(dp), intent(in)  :: kappa, rho, v(3), p, b(3), psi
+    real(dp), intent(out) :: u(9)
+    u(1) = rho
+    u(2:4) = rho*v
+    u(5) = p/(kappa-1.0_dp) + 0.5_dp*rho*sum(v**2) + 0.5_dp*(sum(b**2) + psi**2)
+    u(6:8) = b
+    u(9) = psi
+end subroutine
+
+!! Orszag-Tang-type vortex on the unit cube (GLM-MHD, mu0 = 1):
+!! rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x, 0.1 sin 2 pi z),
+!! B = (-sin 2 pi y, sin 4 pi x, 0)/sqrt(4 pi), psi = 0, plus a deterministic
+!! velocity perturbation of amplitude `noise` (a hash of the node position).
+subroutine fill_orszag_tang(mesh, nn, xi, kappa, noise, state)
+    type(mesh_t), intent(in) :: mesh
+    integer, intent(in)      :: nn
+    real(dp), intent(in)     :: xi(nn), kappa, noise
+    real(dp), intent(out)    :: state(nn, nn, nn, 9, mesh%nelem)
+    real(dp) :: x, y, z, v(3), b(3), h(3)
+    integer(c_int64_t) :: e
+    integer :: i, j, k, d
+    do e = 1, mesh%nelem
+        do k = 1, nn; do j = 1, nn; do i = 1, nn
+            x = mesh%lower(1, e) + 0.5_dp*(xi(i) + 1.0_dp)*mesh%delta(1, e)
+            y = mesh%lower(2, e) + 0.5_dp*(xi(j) + 1.0_dp)*mesh%delta(2, e)
+            z = mesh%lower(3, e) + 0.5_dp*(xi(k) + 1.0_dp)*mesh%delta(3, e)
+            do d = 1, 3
+                h(d) = sin(12.9898_dp*x + 78.233_dp*y + 37.719_dp*z + 17.0_dp*d)*43758.5453_dp
+                h(d) = 2.0_dp*(h(d) - floor(h(d))) - 1.0_dp
+            end do
+            v = (/ -sin(2.0_dp*PI*y), sin(2.0_dp*PI*x), 0.1_dp*sin(2.0_dp*PI*z) /) + noise*h
+            b = (/ -sin(2.0_dp*PI*y), sin(4.0_dp*PI*x), 0.0_dp /)/sqrt(4.0_dp*PI)
+            call prim2cons_mhd(kappa, 25.0_dp/(36.0_dp*PI), v, 5.0_dp/(12.0_dp*PI), b, 0.0_dp, &
+                               state(i, j, k, :, e))
+        end do; end do; end do
+    end do
+end subroutine
+
 !! Raw dump: int64 nelem, int64 n_poly, int64 nvars, then the state array.
 subroutine write_state(path, n_poly, nelem, state)
     character(len=*), intent(in) :: path
@@ -157,7 +198,7 @@ subroutine write_state(path, n_poly, nelem, state)
     real(dp), intent(in)         :: state(:,:,:,:,:)
     integer :: unit
     open(newunit=unit, file=path, access='stream', form='unformatted', status='replace')
-    write(unit) nelem, int(n_poly, c_int64_t), 5_c_int64_t
+    write(unit) nelem, int(n_poly, c_int64_t), int(size(state, 4), c_int64_t)
     write(unit) state
     close(unit)
 end subroutine

@@ -29,8 +29,8 @@ def _rel(a, b):
 
 COMBOS = [
     # (N, volume, surface, indicator, timedisc, setup)
-    (3, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),   # config 1
-    (3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),               # config 2
+    (3, _lib.VOL_WEAK, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "vortex"),   # isentropic vortex scheme
+    (3, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),               # Kelvin-Helmholtz scheme
     (3, _lib.VOL_CENTRAL, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_LSRK54, "vortex"),
     (3, _lib.VOL_EC, _lib.SURF_RUSANOV, _lib.IND_NONE, _lib.TD_SSPRK54, "kh"),
     (3, _lib.VOL_WEAK, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, "kh"),

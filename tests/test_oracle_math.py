@@ -341,7 +341,7 @@ def test_cfl_closed_form():
 
 
 def test_isentropic_vortex_convergence():
-    """Weak-form DGSEM (config 1 operator) converges at about order N+1 on
+    """Weak-form DGSEM (the vortex scheme's operator) converges at about order N+1 on
     the advected isentropic vortex (exact solution: the initial vortex
     translated by u_inf t)."""
     N = 3

@@ -73,7 +73,11 @@ def test_fortran_driver_matches_oracle(tmp_path, setup):
     rc, href = O.run(cfg, m.nbr, m.delta, uref, steps)
     assert rc == 0
     assert np.abs(dts - href).max() <= 1e-13 * href.max()
-    # per variable, relative to its own largest value (psi starts at zero)
-    err = max(float(np.abs(u1[:, h] - uref[:, h]).max() / np.abs(uref[:, h]).max()) for h in range(u0.shape[1]))
+    # per variable, relative to its own largest value (psi grows from zero),
+    # floored at 1e-3 of the state's largest value (the vortex's z momentum
+    # is rounding noise around zero)
+    floor = 1e-3 * np.abs(uref).max()
+    err = max(float(np.abs(u1[:, h] - uref[:, h]).max() / max(np.abs(uref[:, h]).max(), floor))
+              for h in range(u0.shape[1]))
     print(f"  rel Linf after {steps} steps (worst variable) {err:.3e}")
     assert err <= 1e-10

@@ -330,9 +330,12 @@ int build_graph(Batch* b) {
     int rc = launch_step(b);
     if (rc == NEMO_OK) rc = launch_step(b);
     hipError_t e = hipStreamEndCapture(b->stream, &graph);
-    if (rc != NEMO_OK) return rc;
-    if (e != hipSuccess) return fail(NEMO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    for (int i = 0; i < 3; ++i) b->g[i].cur = cur0[i];
+    for (int i = 0; i < 3; ++i) b->g[i].cur = cur0[i];  // capturing ran nothing
+    if (rc != NEMO_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != NEMO_OK) return rc;
+        return fail(NEMO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
     e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(NEMO_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));

@@ -364,7 +364,6 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
     if (cfg->equations == NEMO_EQ_EULER && (cfg->surface < 0 || cfg->surface > 1))
         return fail(NEMO_ERR_ARG, "invalid surface flux for Euler (Rusanov or HLLC)");
     if (cfg->equations == NEMO_EQ_GLM_MHD) {
-        if (cfg->n_poly != 3) return fail(NEMO_ERR_ARG, "GLM-MHD is built for n_poly = 3");
         if (cfg->volume == NEMO_VOL_WEAK)
             return fail(NEMO_ERR_ARG, "GLM-MHD needs flux differencing (NEMO_VOL_CENTRAL or NEMO_VOL_EC)");
         if (cfg->surface != NEMO_SURF_RUSANOV && cfg->surface != NEMO_SURF_HLLD)

@@ -79,15 +79,12 @@ void launch_mortar(Solver& s) {
     mortar_ops(s.ops, &a.op.P[0][0][0], &a.op.R[0][0][0]);
     const dim3 grid(unsigned(s.nmortar)), block(nemo::MortarShape<M>::NT);
     if (s.cfg.equations == NEMO_EQ_GLM_MHD) {
-        // GLM-MHD stages exist for n_poly = 3 only (the others fail after this)
-        if constexpr (M == 4) {
-            if (s.cfg.surface == NEMO_SURF_HLLD)
-                hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_HLLD, Real>), grid, block, 0,
-                                   s.stream, a);
-            else
-                hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_RUSANOV, Real>), grid, block, 0,
-                                   s.stream, a);
-        }
+        if (s.cfg.surface == NEMO_SURF_HLLD)
+            hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_HLLD, Real>), grid, block, 0,
+                               s.stream, a);
+        else
+            hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_MHD, M, nemo::SURF_RUSANOV, Real>), grid, block, 0,
+                               s.stream, a);
     } else if (s.cfg.surface == NEMO_SURF_HLLC) {
         hipLaunchKernelGGL((nemo::mortar_kernel<nemo::EQ_EULER, M, nemo::SURF_HLLC, Real>), grid, block, 0, s.stream, a);
     } else {
@@ -141,12 +138,10 @@ int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, doubl
     NEMO_CASE(nemo::EQ_EULER, nemo::VOL_CENTRAL, nemo::SURF_HLLC)
     NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_RUSANOV)
     NEMO_CASE(nemo::EQ_EULER, nemo::VOL_EC, nemo::SURF_HLLC)
-    if constexpr (M == 4) {
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_HLLD)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_RUSANOV)
-        NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_HLLD)
-    }
+    NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::EQ_MHD, nemo::VOL_CENTRAL, nemo::SURF_HLLD)
+    NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_RUSANOV)
+    NEMO_CASE(nemo::EQ_MHD, nemo::VOL_EC, nemo::SURF_HLLD)
 #undef NEMO_CASE
     return fail(NEMO_ERR_ARG, "unsupported equations/volume/surface/n_poly combination");
 }

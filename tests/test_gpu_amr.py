@@ -413,10 +413,10 @@ def _mhd_state(f, N=3):
     return np.ascontiguousarray(setups.mhd_prim2cons(rho, v, p, B, psi, GAMMA_MHD))
 
 
-def _mhd_cfgs(f, vol, surf, ind, repair=0, precision=0, glm=(0.0, 0.0), td=_lib.TD_LSRK54):
-    g = solver.make_config(N=3, equations=_lib.EQ_GLM_MHD, volume=vol, surface=surf, indicator=ind,
+def _mhd_cfgs(f, vol, surf, ind, repair=0, precision=0, glm=(0.0, 0.0), td=_lib.TD_LSRK54, N=3):
+    g = solver.make_config(N=N, equations=_lib.EQ_GLM_MHD, volume=vol, surface=surf, indicator=ind,
                            timedisc=td, repair=repair, gamma=GAMMA_MHD, precision=precision, mu0=1.0)
-    o = O.with_mortars(O.make_cfg(N=3, eq=O.EQ_MHD, volume=vol, surface=surf, indicator=ind,
+    o = O.with_mortars(O.make_cfg(N=N, eq=O.EQ_MHD, volume=vol, surface=surf, indicator=ind,
                                   timedisc=td, repair=repair, gamma=GAMMA_MHD, mu0=1.0, glm_ch=glm[0],
                                   glm_alpha=glm[1]), f.mortars)
     return g, o
@@ -426,19 +426,20 @@ def _rel_per_var(a, b):
     return [float(np.abs(a[:, h] - b[:, h]).max() / max(np.abs(b[:, h]).max(), 1e-300)) for h in range(a.shape[1])]
 
 
-MHD_COMBOS = [(_lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP), (_lib.VOL_EC, _lib.SURF_RUSANOV, _lib.IND_NONE),
-              (_lib.VOL_CENTRAL, _lib.SURF_HLLD, _lib.IND_NONE), (_lib.VOL_CENTRAL, _lib.SURF_RUSANOV, _lib.IND_PP)]
+MHD_COMBOS = [(3, _lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP), (3, _lib.VOL_EC, _lib.SURF_RUSANOV, _lib.IND_NONE),
+              (3, _lib.VOL_CENTRAL, _lib.SURF_HLLD, _lib.IND_NONE), (3, _lib.VOL_CENTRAL, _lib.SURF_RUSANOV, _lib.IND_PP),
+              (2, _lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP), (4, _lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP)]
 
 
-@pytest.mark.parametrize("combo", MHD_COMBOS, ids=lambda c: f"vol{c[0]}-surf{c[1]}-ind{c[2]}")
+@pytest.mark.parametrize("combo", MHD_COMBOS, ids=lambda c: f"N{c[0]}-vol{c[1]}-surf{c[2]}-ind{c[3]}")
 def test_mhd_amr_rhs_parity(combo):
     """GLM-MHD RHS on the 2:1 forest (every face orientation on mortars,
     GLM speed and damping set) against the oracle, per variable <= 1e-12."""
-    vol, surf, ind = combo
+    N, vol, surf, ind = combo
     f = _forest()
-    u0 = _mhd_state(f)
+    u0 = _mhd_state(f, N)
     glm = (0.8, 4.0)
-    g, o = _mhd_cfgs(f, vol, surf, ind, glm=glm)
+    g, o = _mhd_cfgs(f, vol, surf, ind, glm=glm, N=N)
     print(f"mhd amr rhs parity {combo}: {f.nelem} elements, {len(f.mortars)} mortars", flush=True)
     with solver.Solver(g) as s:
         s.upload_mesh(f)

@@ -18,21 +18,21 @@ RHS_TOL = 1e-12
 RUN_TOL = 1e-10
 
 
-def _cfgs(vol, surf, ind, td=_lib.TD_LSRK54, repair=0, precision=0, glm=(0.0, 0.0)):
-    g = solver.make_config(N=3, equations=_lib.EQ_GLM_MHD, volume=vol, surface=surf, indicator=ind, timedisc=td,
+def _cfgs(vol, surf, ind, td=_lib.TD_LSRK54, repair=0, precision=0, glm=(0.0, 0.0), N=3):
+    g = solver.make_config(N=N, equations=_lib.EQ_GLM_MHD, volume=vol, surface=surf, indicator=ind, timedisc=td,
                            repair=repair, gamma=GAMMA, precision=precision, mu0=1.0)
-    o = O.make_cfg(N=3, eq=O.EQ_MHD, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair,
+    o = O.make_cfg(N=N, eq=O.EQ_MHD, volume=vol, surface=surf, indicator=ind, timedisc=td, repair=repair,
                    gamma=GAMMA, mu0=1.0, glm_ch=glm[0], glm_alpha=glm[1])
     return g, o
 
 
-def _state(m, rough):
+def _state(m, rough, N=3):
     """Orszag-Tang data; `rough` adds density and pressure jumps (and a
     non-zero psi) so that the indicator blends and every term is active."""
-    u = setups.orszag_tang(m, 3, gamma=GAMMA, seed=7)
+    u = setups.orszag_tang(m, N, gamma=GAMMA, seed=7)
     if not rough:
         return u
-    xi, _ = O.lgl(3)
+    xi, _ = O.lgl(N)
     X = m.node_coords(xi)
     rho = u[:, 0] * (1.0 + 0.5 * (X[:, 0] > 0.5))
     v = u[:, 1:4] / u[:, 0:1]
@@ -176,3 +176,52 @@ def test_mhd_fp32_operator_error_per_variable():
         einf = np.abs(u32[:, h] - u64[:, h]).max() / max(np.abs(u64[:, k]).max() for k in group[h])
         print(f"  var {h}: L1 {e1:.2e}  Linf {einf:.2e}")
         assert e1 <= 1e-6, h
+
+
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("combo", [(_lib.VOL_EC, _lib.SURF_HLLD, _lib.IND_PP),
+                                   (_lib.VOL_CENTRAL, _lib.SURF_RUSANOV, _lib.IND_NONE)],
+                         ids=lambda c: f"vol{c[0]}-surf{c[1]}-ind{c[2]}")
+def test_mhd_other_degrees_rhs_and_run(N, combo):
+    """GLM-MHD at n_poly = 2 and 4: RHS per variable <= 1e-12 against the
+    oracle on the rough state (GLM speed and damping set), 20 LSRK steps with
+    repair <= 1e-10 per variable, and the FP32 operator's RHS within 5e-5."""
+    vol, surf, ind = combo
+    m = meshmod.uniform_periodic(4)
+    u0 = _state(m, rough=True, N=N)
+    glm = (0.8, 4.0)
+    g, o = _cfgs(vol, surf, ind, glm=glm, N=N)
+    print(f"mhd N={N} {combo}", flush=True)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        s.set_glm(*glm)
+        r = s.kernel_calc_flux()
+    aref = O.indicator(o, u0)[0] if ind == _lib.IND_PP else None
+    if ind == _lib.IND_PP:
+        assert (aref > 0).any()
+    err = _rel_per_var(r, O.rhs(o, m.nbr, m.delta, u0, aref))
+    print("  rhs rel Linf per var", " ".join(f"{e:.2e}" for e in err))
+    assert max(err) <= RHS_TOL
+    g, o = _cfgs(vol, surf, ind, repair=1, N=N)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        dts = s.run(20, record_dt=True)
+        ug = s.get_state()
+    uo = u0.copy()
+    rc, dto = O.run(o, m.nbr, m.delta, uo, 20)
+    assert rc == 0
+    assert np.abs(dts - dto).max() <= 1e-12 * dto.max()
+    err = _rel_per_var(ug, uo)
+    print("  20 steps rel Linf per var", " ".join(f"{e:.2e}" for e in err))
+    assert max(err) <= RUN_TOL
+    g, o = _cfgs(vol, surf, ind, precision=1, glm=glm, N=N)
+    with solver.Solver(g) as s:
+        s.upload_mesh(m)
+        s.set_state(u0)
+        s.set_glm(*glm)
+        r32 = s.kernel_calc_flux()
+    err = _rel_per_var(r32, O.rhs(o, m.nbr, m.delta, u0, aref))
+    print("  fp32 rhs rel Linf per var", " ".join(f"{e:.2e}" for e in err))
+    assert max(err) <= 5e-5

@@ -148,20 +148,20 @@ def _quad_weights(N, m):
     return W[None, :] * (np.prod(m.delta, axis=1) / 8.0)[:, None]
 
 
-def test_entropy_conservation_of_split_form_mhd_scheme():
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_entropy_conservation_of_split_form_mhd_scheme(N):
     """EC volume flux + Powell/GLM non-conservative terms in split form + EC
     flux and face averages of the non-conservative terms at the faces: the
     semi-discrete entropy change sum J w(u) . rhs vanishes on a periodic mesh
-    (Derigs et al. 2018; Bohm et al. 2018).  Dropping or mis-signing the
-    non-conservative terms, the GLM energy terms or one of the face terms
-    breaks it; Rusanov faces make it strictly negative."""
-    N = 3
+    (Derigs et al. 2018; Bohm et al. 2018), at every degree.  Dropping or
+    mis-signing the non-conservative terms, the GLM energy terms or one of
+    the face terms breaks it; Rusanov faces make it strictly negative."""
     m = meshmod.uniform_periodic(2)
     u = _mhd_rough(m, N, seed=4)
     Wq = _quad_weights(N, m)
-    w = np.stack([[entropy_vars(u[e, :, n]) for n in range(64)] for e in range(m.nelem)])
+    w = np.stack([[entropy_vars(u[e, :, n]) for n in range((N + 1) ** 3)] for e in range(m.nelem)])
     for surf, sign in ((O.SURF_EC_TEST, 0), (O.SURF_RUSANOV, -1)):
-        cfg = _cfg(surface=surf, glm_ch=0.8, glm_alpha=0.0)
+        cfg = _cfg(N=N, surface=surf, glm_ch=0.8, glm_alpha=0.0)
         r = O.rhs(cfg, m.nbr, m.delta, u)
         prod = np.einsum("env,evn->en", w, r)
         dS = (Wq * prod).sum()
@@ -172,17 +172,17 @@ def test_entropy_conservation_of_split_form_mhd_scheme():
             assert dS < -1e-6 * scale
 
 
-def test_glm_damping_and_free_stream():
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_glm_damping_and_free_stream(N):
     """A constant state with psi = 0 is steady; with psi != 0 the only
     non-zero RHS is the damping -glm_alpha psi of the psi equation."""
-    N = 3
     m = meshmod.uniform_periodic(2)
     M3 = (N + 1) ** 3
     for psi, alpha in ((0.0, 3.0), (0.25, 3.0)):
         u0 = prim2cons(1.2, [0.3, -0.2, 0.1], 0.9, [0.4, 0.7, -0.5], psi)
         u = np.ascontiguousarray(np.broadcast_to(u0[None, :, None], (m.nelem, 9, M3)))
         for vol, surf in ((O.VOL_EC, O.SURF_HLLD), (O.VOL_EC, O.SURF_RUSANOV), (O.VOL_CENTRAL, O.SURF_HLLD)):
-            r = O.rhs(_cfg(volume=vol, surface=surf, glm_ch=0.7, glm_alpha=alpha), m.nbr, m.delta, u)
+            r = O.rhs(_cfg(N=N, volume=vol, surface=surf, glm_ch=0.7, glm_alpha=alpha), m.nbr, m.delta, u)
             assert np.allclose(r[:, :8], 0.0, atol=1e-12)
             assert np.allclose(r[:, 8], -alpha * psi, atol=1e-12)
 

@@ -55,6 +55,21 @@ def test_create_rejects_bad_arguments_without_device():
     assert "n_poly" in _lib.last_error()
     cfg = solver.make_config(N=3, gamma=1.0)
     assert _lib.lib().nemo_gpu_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.NEMO_ERR_ARG
+    # combinations that are not built are refused before any device call
+    bad = [
+        (dict(equations=_lib.EQ_GLM_MHD, volume=_lib.VOL_WEAK, surface=_lib.SURF_HLLD), "flux differencing"),
+        (dict(equations=_lib.EQ_GLM_MHD, volume=_lib.VOL_EC, surface=_lib.SURF_HLLC), "GLM-MHD"),
+        (dict(equations=_lib.EQ_GLM_MHD, volume=_lib.VOL_EC, surface=_lib.SURF_HLLD, mu0=0.0), "mu0"),
+        (dict(volume=_lib.VOL_EC, surface=_lib.SURF_HLLD), "Euler"),
+        (dict(equations=_lib.EQ_GLM_MHD, volume=_lib.VOL_EC, surface=_lib.SURF_HLLD, indicator=_lib.IND_MLP),
+         "learned indicator"),
+        (dict(volume=_lib.VOL_WEAK, surface=_lib.SURF_HLLC, indicator=_lib.IND_MLP), "learned indicator"),
+        (dict(precision=2), "precision"),
+    ]
+    for kw, word in bad:
+        cfg = solver.make_config(N=3, **kw)
+        assert _lib.lib().nemo_gpu_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.NEMO_ERR_ARG, kw
+        assert word in _lib.last_error(), (kw, _lib.last_error())
 
 
 def test_operators_entry_point():

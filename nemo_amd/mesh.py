@@ -15,7 +15,15 @@ connectivity into the table the device reads:
     lower[e, d] lower corner of element e
     ijk[e, d]   lattice coordinates of element e
 
-All faces are conforming and aligned; periodic wrap in every direction.
+All faces are conforming and aligned; periodic wrap in every direction, or
+(``with_boundaries``) boundary faces on the sides of the box that are not
+periodic:
+
+    nbr[e, f] = -(b + 1)  face f of e is boundary face b
+    bfaces[b] = (element, face, kind), kind 0 = transmissive, 1 = slip wall
+
+(the BC hook of the reference's face pass, kernel/mcfv/3d/kernel_flux_mod.f90:
+122-226, with the setup's boundary Riemann solvers).
 """
 from __future__ import annotations
 
@@ -38,6 +46,9 @@ def morton_key(i: np.ndarray, j: np.ndarray, k: np.ndarray) -> np.ndarray:
     return _part1by2(i) | (_part1by2(j) << np.uint64(1)) | (_part1by2(k) << np.uint64(2))
 
 
+BC_TRANSMISSIVE, BC_SLIP_WALL = 0, 1
+
+
 @dataclass
 class Mesh:
     shape: tuple            # (nx, ny, nz) elements
@@ -47,6 +58,7 @@ class Mesh:
     lower: np.ndarray       # float64 [nelem, 3]
     ijk: np.ndarray         # int64 [nelem, 3]
     morton: bool
+    bfaces: np.ndarray = None   # int32 [nbound, 3]: element, face, kind (None: periodic)
 
     @property
     def nelem(self) -> int:
@@ -96,3 +108,35 @@ def uniform_periodic(nx: int, ny: int = None, nz: int = None, extent=(1.0, 1.0, 
     lower = np.asarray(origin)[None, :] + ijk * d[None, :]
     return Mesh((nx, ny, nz), tuple(extent), np.ascontiguousarray(nbr), np.ascontiguousarray(delta),
                 np.ascontiguousarray(lower), ijk, use_morton)
+
+
+def with_boundaries(mesh: Mesh, kinds) -> Mesh:
+    """The same box with boundary faces in place of the periodic wrap.
+
+    kinds[f] for the six faces of the box (0 = x-, 1 = x+, ..., 5 = z+):
+    None keeps that side periodic (then its opposite side must be None as
+    well), BC_TRANSMISSIVE or BC_SLIP_WALL makes its element faces boundary
+    faces of that kind."""
+    kinds = list(kinds)
+    if len(kinds) != 6:
+        raise ValueError("one kind per box face")
+    for d in range(3):
+        if (kinds[2 * d] is None) != (kinds[2 * d + 1] is None):
+            raise ValueError("a periodic direction needs both of its sides periodic")
+    n = np.asarray(mesh.shape)
+    nbr = mesh.nbr.copy()
+    rows = []
+    for f, kind in enumerate(kinds):
+        if kind is None:
+            continue
+        d, plus = f // 2, f % 2
+        on = np.nonzero(mesh.ijk[:, d] == (n[d] - 1 if plus else 0))[0]
+        for e in on:
+            rows.append((int(e), f, int(kind)))
+    bfaces = np.array(rows, dtype=np.int32).reshape(-1, 3)
+    # boundary faces in element order, then face order
+    bfaces = bfaces[np.lexsort((bfaces[:, 1], bfaces[:, 0]))] if len(bfaces) else bfaces
+    for b, (e, f, _) in enumerate(bfaces):
+        nbr[e, f] = -(b + 1)
+    return Mesh(mesh.shape, mesh.extent, np.ascontiguousarray(nbr), mesh.delta, mesh.lower, mesh.ijk, mesh.morton,
+                np.ascontiguousarray(bfaces))

@@ -1262,15 +1262,70 @@ void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double 
     }
 }
 
+/* Eulerian slip wall, equations/euler/riemann/eulerian-slip-wall/
+ * riemann_mod.f90:8-38: the wall pressure of the Riemann problem between
+ * the inner state and its mirror image, a rarefaction for v <= 0 (fluid
+ * moving away from the wall, with the reference's floor 1e-5 on the base)
+ * and a shock otherwise. */
+double orc_slip_wall_pressure(const orc_cfg *c, double rho, double v, double p) {
+    const double k = c->gamma;
+    if (v <= 0.0) {
+        const double cs = sqrt(k * p / rho);
+        return p * pow(fmax(1.0 + 0.5 * (k - 1.0) * v / cs, 1e-5), 2.0 * k / (k - 1.0));
+    }
+    const double a = 2.0 / (k + 1.0) / rho, b = (k - 1.0) / (k + 1.0) * p;
+    return p + v / a * 0.5 * (v + sqrt(v * v + 4.0 * a * (p + b)));
+}
+
+/* Boundary faces, the BC hook of kernel_fill_faceflux (kernel/mcfv/3d/
+ * kernel_flux_mod.f90:122-226): the outer state mirrors the inner one
+ * (:123-125) and the face flux is the configured Riemann solver on the pair
+ * (with GLM-MHD's non-conservative term of the inner state), or, for a
+ * slip wall, the wall pressure in the normal momentum (the riemann_north ..
+ * riemann_back aliases of eulerian-slip-wall/riemann_mod.f90:41-200, whose
+ * inner state is rotated so that its normal velocity points at the wall). */
+void orc_boundary_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux) {
+    (void)nelem;
+    const int M = c->N + 1, M2 = M * M, M3 = M2 * M, V = orc_nvars(c);
+    const int mhd = c->eq == ORC_EQ_MHD;
+    const int64_t s0 = 5 * c->nmortar;
+    for (int64_t b = 0; b < c->nbound; ++b) {
+        const int32_t *t = c->bfaces + 3 * b;
+        const int64_t e = t[0];
+        const int f = t[1], kind = t[2], d = f / 2, plus = f % 2;
+        for (int tb = 0; tb < M; ++tb)
+            for (int ta = 0; ta < M; ++ta) {
+                double ui[MAXV], F[MAXV];
+                const int n = node_index(M, d, plus ? M - 1 : 0, ta, tb);
+                for (int h = 0; h < V; ++h) ui[h] = u[e * V * M3 + h * M3 + n];
+                if (kind == ORC_BC_SLIP_WALL && !mhd) {
+                    const double v = (plus ? 1.0 : -1.0) * ui[1 + d] / ui[0];
+                    for (int h = 0; h < V; ++h) F[h] = 0.0;
+                    F[1 + d] = orc_slip_wall_pressure(c, ui[0], v, orc_pressure(c, ui));
+                } else {
+                    phys_riemann(c, c->surface, d, ui, ui, F);
+                    if (mhd) {
+                        double G[MAXV];
+                        phys_nc(c, d, ui, ui, ui, G);
+                        for (int h = 0; h < V; ++h) F[h] = F[h] + G[h];
+                    }
+                }
+                for (int h = 0; h < V; ++h) flux[((s0 + b) * V + h) * M2 + ta + M * tb] = F[h];
+            }
+    }
+}
+
 void orc_rhs(const orc_cfg *c, int64_t nelem, const int32_t *nbr, const double *delta, const double *u,
              const double *alpha, double *rhs) {
     orc_ops op;
     build_ops(c->N, &op);
     const int V = orc_nvars(c);
     double *mflux = NULL;
-    if (c->nmortar > 0) {
-        mflux = (double *)malloc(sizeof(double) * (size_t)(5 * c->nmortar) * (size_t)V * (size_t)op.M2);
-        orc_mortar_fluxes(c, nelem, u, mflux);
+    const int64_t nslots = 5 * c->nmortar + c->nbound;
+    if (nslots > 0) {
+        mflux = (double *)malloc(sizeof(double) * (size_t)nslots * (size_t)V * (size_t)op.M2);
+        if (c->nmortar > 0) orc_mortar_fluxes(c, nelem, u, mflux);
+        if (c->nbound > 0) orc_boundary_fluxes(c, nelem, u, mflux);
     }
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nelem; ++e) {

@@ -54,7 +54,14 @@ typedef struct {
                                -(5m+2+k) (small_k); small_k covers half k&1 of the
                                big face along its first tangential axis and half
                                k>>1 along the second (axes in increasing order) */
+    int64_t nbound;     /* boundary faces (non-periodic meshes) */
+    const int32_t *bfaces;  /* [nbound][3]: element, face, ORC_BC_* kind; face b
+                               carries nbr = -(5 nmortar + b + 1) */
 } orc_cfg;
+
+/* Boundary kinds: transmissive (outer state = inner state, the configured
+ * Riemann solver) and the Eulerian slip wall (Euler only). */
+enum { ORC_BC_TRANSMISSIVE = 0, ORC_BC_SLIP_WALL = 1 };
 
 /* ---- operators (utils/interpol_mod.f90, utils/linalg_mod.f90) ---------- */
 void orc_lgl_nodes_weights(int N, double *x, double *w);
@@ -98,6 +105,11 @@ void orc_mortar_ops(int N, double *P, double *R);
 /* Face fluxes of every mortar slot: flux[slot][var][M*M]; GLM-MHD slots
  * include their side's non-conservative term. */
 void orc_mortar_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux);
+/* Face fluxes of every boundary slot (slots 5 nmortar + b of the same array). */
+void orc_boundary_fluxes(const orc_cfg *c, int64_t nelem, const double *u, double *flux);
+/* Star pressure of the slip wall for inner density rho, pressure p and
+ * velocity v toward the wall. */
+double orc_slip_wall_pressure(const orc_cfg *c, double rho, double v, double p);
 
 /* ---- element operators -------------------------------------------------- */
 int orc_nvars(const orc_cfg *c);

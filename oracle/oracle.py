@@ -48,6 +48,8 @@ class Cfg(ctypes.Structure):
         ("glm_alpha", ctypes.c_double),
         ("nmortar", ctypes.c_int64),
         ("mortars", ctypes.c_void_p),
+        ("nbound", ctypes.c_int64),
+        ("bfaces", ctypes.c_void_p),
     ]
 
 
@@ -56,7 +58,7 @@ def make_cfg(N=3, eq=EQ_EULER, volume=VOL_WEAK, surface=SURF_RUSANOV,
              alpha_max=0.5, alpha_min=0.001, cfl=0.5, mu0=1.0, glm_ch=0.0,
              glm_alpha=0.0) -> Cfg:
     return Cfg(N, eq, volume, surface, indicator, timedisc, repair, 0, gamma,
-               alpha_max, alpha_min, cfl, mu0, glm_ch, glm_alpha, 0, None)
+               alpha_max, alpha_min, cfl, mu0, glm_ch, glm_alpha, 0, None, 0, None)
 
 
 def with_mortars(cfg: Cfg, mortars) -> Cfg:
@@ -66,7 +68,27 @@ def with_mortars(cfg: Cfg, mortars) -> Cfg:
     c._mortars = arr
     c.nmortar = arr.shape[0]
     c.mortars = arr.ctypes.data if arr.shape[0] else None
+    c._bfaces = getattr(cfg, "_bfaces", None)
     return c
+
+
+BC_TRANSMISSIVE, BC_SLIP_WALL = 0, 1
+
+
+def with_boundaries(cfg: Cfg, bfaces) -> Cfg:
+    """Copy of cfg that carries a boundary-face table [nbound][3] = (element,
+    face, kind) (kept alive with the copy)."""
+    c = Cfg.from_buffer_copy(cfg)
+    c._mortars = getattr(cfg, "_mortars", None)
+    arr = np.ascontiguousarray(bfaces if bfaces is not None else np.zeros((0, 3)), dtype=np.int32)
+    c._bfaces = arr
+    c.nbound = arr.shape[0]
+    c.bfaces = arr.ctypes.data if arr.shape[0] else None
+    return c
+
+
+def slip_wall_pressure(cfg: Cfg, rho: float, v: float, p: float) -> float:
+    return lib().orc_slip_wall_pressure(ctypes.byref(cfg), rho, v, p)
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -106,6 +128,9 @@ def lib():
         L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_double, _P, _P]
         L.orc_mortar_ops.argtypes = [ctypes.c_int, _P, _P]
         L.orc_mortar_fluxes.argtypes = [cp, ctypes.c_int64, _P, _P]
+        L.orc_boundary_fluxes.argtypes = [cp, ctypes.c_int64, _P, _P]
+        L.orc_slip_wall_pressure.restype = ctypes.c_double
+        L.orc_slip_wall_pressure.argtypes = [cp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_mhd_pressure.restype = ctypes.c_double
         L.orc_mhd_fastspeed.restype = ctypes.c_double

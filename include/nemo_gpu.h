@@ -104,6 +104,21 @@ int nemo_gpu_upload_mesh(void *handle, int64_t nelem, const int32_t *nbr, const 
 int nemo_gpu_upload_mesh_amr(void *handle, int64_t nelem, const int32_t *nbr, const double *delta,
                              int64_t nmortar, const int32_t *mortars);
 
+/* Non-periodic mesh: as nemo_gpu_upload_mesh_amr plus the boundary sides of
+ * mesh_alloc (side%boundary > 0, mesh/p4est/mesh_utils_mod.f90:340-376) and
+ * the BC hook of the face pass (kernel/mcfv/3d/kernel_flux_mod.f90:122-226).
+ * bfaces[b] = (element, face, kind); boundary face b carries
+ * nbr = -(5 nmortar + b + 1).  kind 0 = transmissive (the outer state mirrors
+ * the inner one, :123-125, and the configured Riemann solver gives the flux;
+ * GLM-MHD adds the inner state's non-conservative term), 1 = slip wall (the
+ * reference's eulerian-slip-wall boundary solver,
+ * equations/euler/riemann/eulerian-slip-wall/riemann_mod.f90:8-200; Euler
+ * only). */
+#define NEMO_BC_TRANSMISSIVE 0
+#define NEMO_BC_SLIP_WALL 1
+int nemo_gpu_upload_mesh_bc(void *handle, int64_t nelem, const int32_t *nbr, const double *delta,
+                            int64_t nmortar, const int32_t *mortars, int64_t nbound, const int32_t *bfaces);
+
 /* Mesh adaptation: replaces the state-transfer half of mesh_adapt_wrapped
  * (mesh/p4est/mesh_utils_mod.f90:966-1000) with kernel_amr_refine /
  * kernel_amr_coarsen (kernel/mcfv/3d/kernel_amr_mod.f90:12-70).  The new

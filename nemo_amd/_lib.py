@@ -16,6 +16,7 @@ NEMO_ERR_STATE = 3
 NEMO_ERR_NOT_READY = 4
 
 EQ_EULER, EQ_GLM_MHD = 0, 1
+BC_TRANSMISSIVE, BC_SLIP_WALL = 0, 1
 VOL_WEAK, VOL_CENTRAL, VOL_EC = 0, 1, 2
 SURF_RUSANOV, SURF_HLLC, SURF_HLLD = 0, 1, 2
 IND_NONE, IND_PP, IND_MLP = 0, 1, 2
@@ -61,6 +62,7 @@ SIGNATURES = {
     "nemo_gpu_destroy": [_VP],
     "nemo_gpu_upload_mesh": [_VP, ctypes.c_int64, _PI32, _P],
     "nemo_gpu_upload_mesh_amr": [_VP, ctypes.c_int64, _PI32, _P, ctypes.c_int64, _PI32],
+    "nemo_gpu_upload_mesh_bc": [_VP, ctypes.c_int64, _PI32, _P, ctypes.c_int64, _PI32, ctypes.c_int64, _PI32],
     "nemo_gpu_remesh": [_VP, ctypes.c_int64, _PI32, _P, ctypes.c_int64, _PI32, ctypes.c_int64, _PI32,
                         ctypes.POINTER(ctypes.c_int64)],
     "nemo_gpu_upload_state": [_VP, _P],

@@ -1,5 +1,5 @@
-// Non-conforming (2:1) faces and the state transfer of mesh adaptation
-// (gfx950).
+// Non-conforming (2:1) faces, boundary faces and the state transfer of mesh
+// adaptation (gfx950).
 //
 // mortar_kernel: the reference's mortar sequence of
 // kernel_fill_faceflux_cb (kernel/dgsem/weakform/2d/kernel_flux_mod.f90:
@@ -180,6 +180,101 @@ __global__ void __launch_bounds__(MortarShape<M>::NT) mortar_kernel(MortarArgs<M
             out[h * M2] = acc;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary faces of non-periodic meshes
+// ---------------------------------------------------------------------------
+
+enum { BC_TRANSMISSIVE = 0, BC_SLIP_WALL = 1 };
+
+template <int M>
+struct BoundaryArgs {
+    const double* __restrict__ u_in;     // [var][elem][node]
+    double* __restrict__ flux;           // [slot][var][M*M]
+    const int32_t* __restrict__ bfaces;  // [nbound][3]: element, face, kind
+    const double* __restrict__ glm;      // GLM-MHD: c_h of the current step in glm[0]
+    long long nelem, nbound, slot0;      // slot0 = 5 nmortar: the boundary slots follow the mortars'
+    Phys64 ph64;
+};
+
+// Wall pressure of the Eulerian slip wall (equations/euler/riemann/
+// eulerian-slip-wall/riemann_mod.f90:8-38): the Riemann problem between the
+// inner state and its mirror image, a rarefaction for v <= 0 (v: velocity
+// toward the wall; the base floored at 1e-5 as there) and a shock otherwise.
+__device__ __forceinline__ double slip_wall_pressure(double kappa, double rho, double v, double p) {
+    if (v <= 0.0) {
+        const double c = sqrt(kappa * p / rho);
+        return p * pow(fmax(1.0 + 0.5 * (kappa - 1.0) * v / c, 1e-5), 2.0 * kappa / (kappa - 1.0));
+    }
+    const double a = 2.0 / (kappa + 1.0) / rho, b = (kappa - 1.0) / (kappa + 1.0) * p;
+    return p + v / a * 0.5 * (v + sqrt(v * v + 4.0 * a * (p + b)));
+}
+
+// Face flux of one boundary face node in direction D, FP64: the configured
+// Riemann solver between the inner state and its copy (transmissive; for
+// GLM-MHD with the inner state's non-conservative term), or the wall
+// pressure in the normal momentum (slip wall, Euler).
+template <int EQ, int SURF, int D>
+__device__ __forceinline__ void boundary_flux_d(const Phys64& P, double ch, const double* ui, int kind, bool plus,
+                                                double* f) {
+    if constexpr (EQ == EQ_MHD) {
+        constexpr int BN = Mag<D, 0>::idx;
+        MhdGas<double> mg = P.mhd;
+        mg.ch = ch;
+        MhdState<double> S;
+#pragma unroll
+        for (int h = 0; h < 9; ++h) S.u[h] = ui[h];
+        mhd_make_state<false>(mg, S);
+        if constexpr (SURF == SURF_HLLD)
+            mhd_hlld<D>(mg, S, S, f);
+        else
+            mhd_rusanov<D>(mg, S, S, f);
+        mhd_nc_add<D>(mg, S, S.u[BN], S.u[GLMP], f);
+    } else {
+        if (kind == BC_SLIP_WALL) {
+            const double v = (plus ? 1.0 : -1.0) * ui[1 + D] / ui[DENS];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) f[h] = 0.0;
+            f[1 + D] = slip_wall_pressure(P.gas.kappa, ui[DENS], v, pressure(P.gas, ui));
+        } else {
+            NodeState<double> S;
+#pragma unroll
+            for (int h = 0; h < 5; ++h) S.u[h] = ui[h];
+            make_state<D, false>(P.gas, S);
+            riemann<SURF, D>(P.gas, S, S, f);
+        }
+    }
+}
+
+// One lane per boundary face node: the inner node's state, the face flux,
+// the face's slot (read by the stage kernel as for a mortar face).
+template <int EQ, int M, int SURF>
+__global__ void __launch_bounds__(64) boundary_kernel(BoundaryArgs<M> a) {
+    constexpr int M2 = M * M, M3 = M2 * M, V = NVars<EQ>::V;
+    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (t >= a.nbound * M2) return;
+    const long long b = t / M2;
+    const int q = int(t - b * M2), ta = q % M, tb = q / M;
+    const int32_t* bf = a.bfaces + b * 3;
+    const long long e = bf[0];
+    const int face = bf[1], kind = bf[2], d = face >> 1;
+    const bool plus = (face & 1) != 0;
+    const long long EM3 = a.nelem * (long long)M3;
+    const int n = face_node<M>(d, plus ? M - 1 : 0, ta, tb);
+    double ui[V], f[V];
+#pragma unroll
+    for (int h = 0; h < V; ++h) ui[h] = a.u_in[h * EM3 + e * M3 + n];
+    const double ch = EQ == EQ_MHD ? a.glm[0] : 0.0;
+    if (d == 0)
+        boundary_flux_d<EQ, SURF, 0>(a.ph64, ch, ui, kind, plus, f);
+    else if (d == 1)
+        boundary_flux_d<EQ, SURF, 1>(a.ph64, ch, ui, kind, plus, f);
+    else
+        boundary_flux_d<EQ, SURF, 2>(a.ph64, ch, ui, kind, plus, f);
+    double* out = a.flux + size_t(a.slot0 + b) * V * M2 + q;
+#pragma unroll
+    for (int h = 0; h < V; ++h) out[h * M2] = f[h];
 }
 
 // ---------------------------------------------------------------------------

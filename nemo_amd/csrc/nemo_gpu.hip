@@ -72,9 +72,13 @@ void free_mesh_buffers(Solver& s) {
     s.cap = 0;
     (void)hipFree(s.mortars);
     (void)hipFree(s.mortar_flux);
+    (void)hipFree(s.bfaces);
     s.mortars = nullptr;
     s.mortar_flux = nullptr;
+    s.bfaces = nullptr;
     s.nmortar = 0;
+    s.nbound = 0;
+    s.nslots = 0;
 }
 
 
@@ -199,22 +203,36 @@ struct DeviceBuffer {
 };
 
 // Mesh checks shared by the uploads: conforming neighbours in range, every
-// non-conforming face pointing at the one mortar slot that belongs to it.
+// non-conforming or boundary face pointing at the one face slot that
+// belongs to it (mortar slots first, then the boundary faces').
 int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
-               const int32_t* mortars) {
+               const int32_t* mortars, int64_t nbound = 0, const int32_t* bfaces = nullptr) {
     if (!nbr || !delta || nelem <= 0 || nelem > INT32_MAX) return fail(NEMO_ERR_ARG, "invalid mesh arguments");
     if (nmortar < 0 || (nmortar > 0 && !mortars) || nmortar > INT32_MAX / 5)
         return fail(NEMO_ERR_ARG, "invalid mortar arguments");
-    std::vector<int32_t> owner(size_t(5 * nmortar), -1);
+    if (nbound < 0 || (nbound > 0 && !bfaces) || nbound > INT32_MAX - 5 * nmortar)
+        return fail(NEMO_ERR_ARG, "invalid boundary arguments");
+    const int64_t nslots = 5 * nmortar + nbound;
+    std::vector<int32_t> owner(size_t(nslots), -1);
     for (int64_t i = 0; i < 6 * nelem; ++i) {
         const int32_t n = nbr[i];
         if (n >= nelem) return fail(NEMO_ERR_ARG, "neighbour index out of range");
         if (n < 0) {
             const int64_t slot = -int64_t(n) - 1;
-            if (slot >= 5 * nmortar) return fail(NEMO_ERR_ARG, "mortar slot out of range");
-            if (owner[size_t(slot)] != -1) return fail(NEMO_ERR_ARG, "mortar slot used by two faces");
+            if (slot >= nslots) return fail(NEMO_ERR_ARG, "face slot out of range");
+            if (owner[size_t(slot)] != -1) return fail(NEMO_ERR_ARG, "face slot used by two faces");
             owner[size_t(slot)] = int32_t(i);
         }
+    }
+    for (int64_t b = 0; b < nbound; ++b) {
+        const int32_t* t = bfaces + 3 * b;
+        if (t[0] < 0 || t[0] >= nelem || t[1] < 0 || t[1] > 5) return fail(NEMO_ERR_ARG, "boundary face out of range");
+        if (t[2] != nemo::BC_TRANSMISSIVE && t[2] != nemo::BC_SLIP_WALL)
+            return fail(NEMO_ERR_ARG, "unknown boundary kind");
+        if (t[2] == nemo::BC_SLIP_WALL && s.cfg.equations != NEMO_EQ_EULER)
+            return fail(NEMO_ERR_ARG, "the slip wall is built for the Euler equations");
+        if (owner[size_t(5 * nmortar + b)] != t[0] * 6 + t[1])
+            return fail(NEMO_ERR_ARG, "boundary face does not match nbr");
     }
     for (int64_t m = 0; m < nmortar; ++m) {
         const int32_t* t = mortars + 6 * m;
@@ -236,7 +254,7 @@ int check_mesh(const Solver& s, int64_t nelem, const int32_t* nbr, const double*
 // The solver counts as having no mesh until every copy is in place, so a
 // failure part way leaves no entry point running on half-built buffers.
 int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
-             const int32_t* mortars) {
+             const int32_t* mortars, int64_t nbound = 0, const int32_t* bfaces = nullptr) {
     HIP_TRY(hipStreamSynchronize(s.stream));
     s.have_mesh = false;
     if (nelem != s.nelem) {
@@ -268,20 +286,31 @@ int set_mesh(Solver& s, int64_t nelem, const int32_t* nbr, const double* delta, 
     }
     if (nmortar != s.nmortar) {
         (void)hipFree(s.mortars);
-        (void)hipFree(s.mortar_flux);
         s.mortars = nullptr;
-        s.mortar_flux = nullptr;
         s.nmortar = nmortar;
-        if (nmortar > 0) {
-            HIP_TRY(hipMalloc(&s.mortars, size_t(nmortar) * 6 * sizeof(int32_t)));
-            HIP_TRY(hipMalloc(&s.mortar_flux, size_t(nmortar) * 5 * size_t(s.V) * size_t(s.M * s.M) * sizeof(double)));
-        }
+        if (nmortar > 0) HIP_TRY(hipMalloc(&s.mortars, size_t(nmortar) * 6 * sizeof(int32_t)));
+    }
+    if (nbound != s.nbound) {
+        (void)hipFree(s.bfaces);
+        s.bfaces = nullptr;
+        s.nbound = nbound;
+        if (nbound > 0) HIP_TRY(hipMalloc(&s.bfaces, size_t(nbound) * 3 * sizeof(int32_t)));
+    }
+    if (5 * nmortar + nbound != s.nslots) {
+        (void)hipFree(s.mortar_flux);
+        s.mortar_flux = nullptr;
+        s.nslots = 5 * nmortar + nbound;
+        if (s.nslots > 0)
+            HIP_TRY(hipMalloc(&s.mortar_flux, size_t(s.nslots) * size_t(s.V) * size_t(s.M * s.M) * sizeof(double)));
     }
     if (!s.counter) HIP_TRY(hipMalloc(&s.counter, sizeof(int)));
     HIP_TRY(hipMemcpyAsync(s.nbr, nbr, size_t(nelem) * 6 * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
     HIP_TRY(hipMemcpyAsync(s.delta, delta, size_t(nelem) * 3 * sizeof(double), hipMemcpyHostToDevice, s.stream));
     if (nmortar > 0)
         HIP_TRY(hipMemcpyAsync(s.mortars, mortars, size_t(nmortar) * 6 * sizeof(int32_t), hipMemcpyHostToDevice,
+                               s.stream));
+    if (nbound > 0)
+        HIP_TRY(hipMemcpyAsync(s.bfaces, bfaces, size_t(nbound) * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
                                s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));
     s.have_mesh = true;
@@ -445,10 +474,15 @@ int nemo_gpu_upload_mesh(void* handle, int64_t nelem, const int32_t* nbr, const 
 
 int nemo_gpu_upload_mesh_amr(void* handle, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
                              const int32_t* mortars) {
+    return nemo_gpu_upload_mesh_bc(handle, nelem, nbr, delta, nmortar, mortars, 0, nullptr);
+}
+
+int nemo_gpu_upload_mesh_bc(void* handle, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,
+                            const int32_t* mortars, int64_t nbound, const int32_t* bfaces) {
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail(NEMO_ERR_ARG, "null handle");
-    if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars)) return rc;
-    return set_mesh(*s, nelem, nbr, delta, nmortar, mortars);
+    if (int rc = check_mesh(*s, nelem, nbr, delta, nmortar, mortars, nbound, bfaces)) return rc;
+    return set_mesh(*s, nelem, nbr, delta, nmortar, mortars, nbound, bfaces);
 }
 
 int nemo_gpu_remesh(void* handle, int64_t nelem, const int32_t* nbr, const double* delta, int64_t nmortar,

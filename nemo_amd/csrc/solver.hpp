@@ -33,7 +33,10 @@ struct Solver {
     double* delta = nullptr;
     long long nmortar = 0;             // non-conforming (2:1) faces
     int32_t* mortars = nullptr;        // [nmortar][6]
-    double* mortar_flux = nullptr;     // [5*nmortar][5][M*M]
+    long long nbound = 0;              // boundary faces (non-periodic meshes)
+    int32_t* bfaces = nullptr;         // [nbound][3]: element, face, kind
+    long long nslots = 0;              // 5*nmortar + nbound face slots
+    double* mortar_flux = nullptr;     // [nslots][V][M*M]: mortar slots, then boundary slots
     int* counter = nullptr;            // remap fallbacks
     int32_t* units = nullptr;          // replacement units of the last remesh
     long long units_cap = 0;

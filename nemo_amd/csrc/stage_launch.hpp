@@ -92,12 +92,40 @@ void launch_mortar(Solver& s) {
     }
 }
 
+// Fluxes of the boundary faces of the stage input (FP64 for either operator).
+template <int M>
+void launch_boundary(Solver& s) {
+    nemo::BoundaryArgs<M> a{};
+    a.u_in = s.u[s.cur];
+    a.flux = s.mortar_flux;
+    a.bfaces = s.bfaces;
+    a.glm = s.dt_dev + 3;
+    a.nelem = s.nelem;
+    a.nbound = s.nbound;
+    a.slot0 = 5 * s.nmortar;
+    a.ph64 = make_phys64(s.cfg);
+    const dim3 grid{unsigned((s.nbound * M * M + 63) / 64)}, block{64};
+    if (s.cfg.equations == NEMO_EQ_GLM_MHD) {
+        if (s.cfg.surface == NEMO_SURF_HLLD)
+            hipLaunchKernelGGL((nemo::boundary_kernel<nemo::EQ_MHD, M, nemo::SURF_HLLD>), grid, block, 0, s.stream, a);
+        else
+            hipLaunchKernelGGL((nemo::boundary_kernel<nemo::EQ_MHD, M, nemo::SURF_RUSANOV>), grid, block, 0, s.stream,
+                               a);
+    } else if (s.cfg.surface == NEMO_SURF_HLLC) {
+        hipLaunchKernelGGL((nemo::boundary_kernel<nemo::EQ_EULER, M, nemo::SURF_HLLC>), grid, block, 0, s.stream, a);
+    } else {
+        hipLaunchKernelGGL((nemo::boundary_kernel<nemo::EQ_EULER, M, nemo::SURF_RUSANOV>), grid, block, 0, s.stream,
+                           a);
+    }
+}
+
 template <int M, typename Real>
 int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, double* rhs_out, double* alpha_out,
                     double* cfl_part, const LaunchOpts& o) {
     const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
     if (s.nmortar > 0) launch_mortar<M, Real>(s);
+    if (s.nbound > 0) launch_boundary<M>(s);
     if constexpr (M == 4) {
         // learned indicator, and the training-data pass of any Euler scheme
         if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {

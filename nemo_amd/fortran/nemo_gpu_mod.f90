@@ -21,6 +21,8 @@ integer(c_int), parameter :: NEMO_ERR_NOT_READY = 4
 
 integer(c_int32_t), parameter :: NEMO_EQ_EULER     = 0
 integer(c_int32_t), parameter :: NEMO_EQ_GLM_MHD   = 1
+integer(c_int32_t), parameter :: NEMO_BC_TRANSMISSIVE = 0
+integer(c_int32_t), parameter :: NEMO_BC_SLIP_WALL    = 1
 integer(c_int32_t), parameter :: NEMO_VOL_WEAK     = 0
 integer(c_int32_t), parameter :: NEMO_VOL_CENTRAL  = 1
 integer(c_int32_t), parameter :: NEMO_VOL_EC       = 2
@@ -85,6 +87,19 @@ interface
         type(c_ptr), value                :: handle
         integer(c_int64_t), value         :: nelem, nmortar
         integer(c_int32_t), intent(in)    :: nbr(*), mortars(*)
+        real(c_double), intent(in)        :: delta(*)
+        integer(c_int)                    :: rc
+    end function
+
+    !! non-periodic mesh: boundary sides as bfaces(3, nbound) = (quad, its
+    !! face, kind: NEMO_BC_TRANSMISSIVE or NEMO_BC_SLIP_WALL), with
+    !! nbr = -(5*nmortar + b + 1) on boundary face b (0-based)
+    function nemo_gpu_upload_mesh_bc(handle, nelem, nbr, delta, nmortar, mortars, nbound, bfaces) result(rc) &
+            bind(c, name='nemo_gpu_upload_mesh_bc')
+        import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+        type(c_ptr), value                :: handle
+        integer(c_int64_t), value         :: nelem, nmortar, nbound
+        integer(c_int32_t), intent(in)    :: nbr(*), mortars(*), bfaces(*)
         real(c_double), intent(in)        :: delta(*)
         integer(c_int)                    :: rc
     end function

@@ -107,12 +107,16 @@ class Solver:
 
     # -- mesh and state (mesh_alloc / payload upload) -----------------------
     def upload_mesh(self, mesh: Mesh) -> None:
-        """Connectivity of a uniform mesh or of an adaptive forest (nemo_amd.amr.Forest)."""
+        """Connectivity of a uniform mesh (periodic, or with the boundary faces
+        of nemo_amd.mesh.with_boundaries) or of an adaptive forest
+        (nemo_amd.amr.Forest)."""
         nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32)
         delta = np.ascontiguousarray(mesh.delta, dtype=np.float64)
         mortars = np.ascontiguousarray(getattr(mesh, "mortars", np.zeros((0, 6))), dtype=np.int32)
-        self._call("nemo_gpu_upload_mesh_amr", self._h, ctypes.c_int64(mesh.nelem), iptr(nbr), dptr(delta),
-                   ctypes.c_int64(mortars.shape[0]), iptr(mortars))
+        bf = getattr(mesh, "bfaces", None)
+        bfaces = np.ascontiguousarray(bf if bf is not None else np.zeros((0, 3)), dtype=np.int32)
+        self._call("nemo_gpu_upload_mesh_bc", self._h, ctypes.c_int64(mesh.nelem), iptr(nbr), dptr(delta),
+                   ctypes.c_int64(mortars.shape[0]), iptr(mortars), ctypes.c_int64(bfaces.shape[0]), iptr(bfaces))
         self.nelem = mesh.nelem
 
     def remesh(self, forest, units: np.ndarray) -> int:

@@ -28,8 +28,9 @@ typedef struct {
     int32_t lrefine_min;          /* mesh%minlevel + 1 */
     int32_t lrefine_max;          /* mesh%maxlevel + 1 */
     int32_t file_format_version;  /* 9 (checkpoint_types_mod.f90:61) */
-    int32_t periodic;             /* boundary type strings: periodic or outflow */
-    int32_t pad;
+    int8_t boundary[6];           /* box sides x-, x+, y-, y+, z-, z+: 0 periodic, 1 outflow,
+                                     2 reflecting (the xl_boundary_type .. zr_boundary_type strings) */
+    int16_t pad;
     double time, dt, next_checkpoint_time;
     double xmin, xmax, ymin, ymax, zmin, zmax;
     double cfl, gamma, tinitial, tmax, checkpoint_interval_time;

@@ -195,10 +195,10 @@ int write_file(const char* path, const nemo_flash_meta* m, const nemo_flash_tree
     std::vector<std::pair<std::string, int>> lpar = {{"restart", 0}};
     std::vector<std::pair<std::string, std::string>> spar = {
         {"unitsystem", "none"}, {"geometry", "cartesian"}, {"basen", "chkpt_"}};
-    const char* bnd = m->periodic ? "periodic" : "outflow";
-    for (const char* k : {"xl_boundary_type", "xr_boundary_type", "yl_boundary_type", "yr_boundary_type",
-                          "zl_boundary_type", "zr_boundary_type"})
-        spar.emplace_back(k, bnd);
+    const char* const bkey[6] = {"xl_boundary_type", "xr_boundary_type", "yl_boundary_type",
+                                 "yr_boundary_type", "zl_boundary_type", "zr_boundary_type"};
+    const char* const bname[3] = {"periodic", "outflow", "reflecting"};
+    for (int side = 0; side < 6; ++side) spar.emplace_back(bkey[side], bname[int(m->boundary[side])]);
     std::vector<std::pair<std::string, int>> iscal = {
         {"nstep", m->nstep}, {"nxb", m->nxb}, {"nyb", m->nxb}, {"nzb", m->nxb}, {"dimensionality", 3},
         {"globalnumblocks", int(t->numblocks)}, {"checkpointfilenumber", m->file_number}};
@@ -270,6 +270,9 @@ int nemo_flash_write(const char* path, const nemo_flash_meta* meta, const nemo_f
         !tree->which_child || tree->numblocks <= 0 || tree->nleaves <= 0 || tree->nleaves > tree->numblocks ||
         meta->nxb <= 0)
         return fail("invalid checkpoint arguments");
+    for (int side = 0; side < 6; ++side)
+        if (meta->boundary[side] < 0 || meta->boundary[side] > 2)
+            return fail("boundary type of a box side out of range");
     const std::string tmp = std::string(path) + ".partial";
     H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
     int rc = write_file(tmp.c_str(), meta, tree, nunk, unknown_names, leaf_data, leaf_blend);

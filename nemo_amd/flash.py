@@ -31,7 +31,7 @@ MHD_NAMES = EULER_NAMES + ["magx", "magy", "magz", "glmp"]
 class Meta(ctypes.Structure):
     _fields_ = [("nxb", ctypes.c_int32), ("nstep", ctypes.c_int32), ("file_number", ctypes.c_int32),
                 ("lrefine_min", ctypes.c_int32), ("lrefine_max", ctypes.c_int32),
-                ("file_format_version", ctypes.c_int32), ("periodic", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("file_format_version", ctypes.c_int32), ("boundary", ctypes.c_int8 * 6), ("pad", ctypes.c_int16),
                 ("time", ctypes.c_double), ("dt", ctypes.c_double), ("next_checkpoint_time", ctypes.c_double),
                 ("xmin", ctypes.c_double), ("xmax", ctypes.c_double), ("ymin", ctypes.c_double),
                 ("ymax", ctypes.c_double), ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
@@ -117,6 +117,13 @@ def write(path: str, forest, N: int, u: np.ndarray, *, time: float = 0.0, dt: fl
     """Write the state u [nelem, V, M^3] (element order of the forest) as a
     FLASH checkpoint; a uniform power-of-two mesh is taken as a forest of one
     level.  Returns the path."""
+    # boundary type of each box side: periodic, or outflow / reflecting for
+    # the transmissive / slip-wall faces of a box with boundary faces
+    sides = [0] * 6
+    bf = getattr(forest, "bfaces", None)
+    if bf is not None:
+        for _, f, kind in np.asarray(bf).reshape(-1, 3):
+            sides[int(f)] = 2 if int(kind) == 1 else 1
     if not isinstance(forest, amr.Forest):
         n = forest.shape[0]
         if not (forest.morton and n & (n - 1) == 0 and forest.shape == (n, n, n)):
@@ -140,7 +147,8 @@ def write(path: str, forest, N: int, u: np.ndarray, *, time: float = 0.0, dt: fl
     dtmin_arr = None if dtmin is None else np.ascontiguousarray(dtmin, dtype=np.float64)
     lo, hi = forest.origin, tuple(o + e for o, e in zip(forest.origin, forest.extent))
     meta = Meta(nxb=M, nstep=nstep, file_number=number, lrefine_min=forest.minlevel + 1,
-                lrefine_max=forest.maxlevel + 1, file_format_version=9, periodic=1, pad=0, time=time, dt=dt,
+                lrefine_max=forest.maxlevel + 1, file_format_version=9, boundary=(ctypes.c_int8 * 6)(*sides), pad=0,
+                time=time, dt=dt,
                 next_checkpoint_time=0.0, xmin=lo[0], xmax=hi[0], ymin=lo[1], ymax=hi[1], zmin=lo[2], zmax=hi[2],
                 cfl=cfl, gamma=gamma, tinitial=0.0, tmax=tmax, checkpoint_interval_time=0.0)
     tree = Tree(numblocks=t.numblocks, nleaves=t.nleaves, coordinates=coords.ctypes.data,

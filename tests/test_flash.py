@@ -97,3 +97,24 @@ def test_uniform_mesh_checkpoint_is_full_tree(tmp_path):
     assert lev.tolist() == [1] + [2] * 8 + [3] * 64
     d = _dump_binary(path, "/momy", str(tmp_path), "<f8").reshape(nb, 27)
     assert np.array_equal(d[9:], u[:, 2])
+
+
+def test_boundary_types_per_box_side(tmp_path):
+    """A box with boundary faces records each side's boundary type: periodic,
+    outflow (transmissive faces) or reflecting (slip walls)."""
+    m = meshmod.with_boundaries(meshmod.uniform_periodic(4), [meshmod.BC_TRANSMISSIVE, meshmod.BC_TRANSMISSIVE,
+                                                              meshmod.BC_SLIP_WALL, meshmod.BC_SLIP_WALL, None, None])
+    u = np.random.default_rng(2).random((m.nelem, 5, 27))
+    path = flash.write(str(tmp_path / "b.f5"), m, 2, u)
+    txt = subprocess.run([H5DUMP, "-d", "/string runtime parameters", path], check=True, capture_output=True,
+                         text=True).stdout
+    found = {}
+    for side in ("xl", "xr", "yl", "yr", "zl", "zr"):
+        i = txt.index(f"{side}_boundary_type")
+        found[side] = txt[i:i + 120].split('"')[2]
+    assert found == {"xl": "outflow", "xr": "outflow", "yl": "reflecting", "yr": "reflecting",
+                     "zl": "periodic", "zr": "periodic"}, found
+    path = flash.write(str(tmp_path / "p.f5"), meshmod.uniform_periodic(4), 2, u)
+    txt = subprocess.run([H5DUMP, "-d", "/string runtime parameters", path], check=True, capture_output=True,
+                         text=True).stdout
+    assert txt.count('"periodic') == 6

@@ -21,6 +21,8 @@
 !!          | ot     (Orszag-Tang, GLM-MHD: EC flux differencing, HLLD,
 !!                    PP blending, LSRK 5-4, positivity repair; the GLM
 !!                    speed and damping follow each step's CFL reduction)
+!!          | sod    (Sod's shock tube between slip walls across x, periodic
+!!                    in y and z: the kh scheme with boundary faces)
 !! Writes PREFIX.init.bin, PREFIX.final.bin (nemo_setup_mod::write_state)
 !! and PREFIX.dt.txt (one time step per line).
 !! ========================================================================== !!
@@ -58,6 +60,8 @@ type(mesh_t) :: mesh
 type(nemo_gpu_config) :: cfg
 type(c_ptr) :: h
 real(dp), allocatable :: state(:,:,:,:,:), xi(:), wi(:), dmat(:), vinv(:), dts(:)
+integer(c_int32_t), allocatable :: bfaces(:,:), no_mortars(:)
+integer(c_int64_t) :: nbound
 character(len=1024) :: setup, prefix, arg
 integer :: n, n_poly, nsteps, nn, nvars, unit, it
 integer(c_int) :: rc
@@ -87,6 +91,12 @@ case ('vortex')
     cfg%timedisc = NEMO_TD_SSPRK54
     cfg%repair = 0
 case ('kh')
+    cfg%volume = NEMO_VOL_EC
+    cfg%surface = NEMO_SURF_HLLC
+    cfg%indicator = NEMO_IND_PP
+    cfg%timedisc = NEMO_TD_LSRK54
+    cfg%repair = 1
+case ('sod')
     cfg%volume = NEMO_VOL_EC
     cfg%surface = NEMO_SURF_HLLC
     cfg%indicator = NEMO_IND_PP
@@ -151,7 +161,14 @@ subroutine hook_init()
     call check(nemo_gpu_operators(int(n_poly, c_int32_t), xi, wi, dmat, vinv), 'nemo_gpu_operators')
     call check(nemo_gpu_create(cfg, h), 'nemo_gpu_create')
     call mesh_uniform(n, mesh)
-    call check(nemo_gpu_upload_mesh(h, mesh%nelem, mesh%nbr, mesh%delta), 'nemo_gpu_upload_mesh')
+    if (trim(setup) == 'sod') then
+        call mesh_walls_x(mesh, NEMO_BC_SLIP_WALL, nbound, bfaces)
+        allocate(no_mortars(6))
+        call check(nemo_gpu_upload_mesh_bc(h, mesh%nelem, mesh%nbr, mesh%delta, 0_c_int64_t, no_mortars, &
+                                           nbound, bfaces), 'nemo_gpu_upload_mesh_bc')
+    else
+        call check(nemo_gpu_upload_mesh(h, mesh%nelem, mesh%nbr, mesh%delta), 'nemo_gpu_upload_mesh')
+    end if
     nvars = merge(9, 5, cfg%equations == NEMO_EQ_GLM_MHD)
     allocate(state(nn, nn, nn, nvars, mesh%nelem))
     select case (trim(setup))
@@ -159,6 +176,8 @@ subroutine hook_init()
         call fill_vortex(mesh, nn, xi, cfg%gamma, state)
     case ('kh')
         call fill_kh(mesh, nn, xi, cfg%gamma, 0.01_dp, state)
+    case ('sod')
+        call fill_sod(mesh, nn, xi, cfg%gamma, state)
     case default
         call fill_orszag_tang(mesh, nn, xi, cfg%gamma, 1.0e-3_dp, state)
     end select

@@ -149,6 +149,56 @@ subroutine fill_kh(mesh, nn, xi, kappa, noise, state)
     end do
 end subroutine
 
+!! Boundary faces across x (the mesh stays periodic in y and z): the x- face
+!! of every element in the first layer and the x+ face of every element in
+!! the last one, in element then face order, each carrying
+!! nbr = -(b + 1) for its boundary face b (0-based) and the given kind.
+subroutine mesh_walls_x(mesh, kind, nbound, bfaces)
+    type(mesh_t), intent(inout)      :: mesh
+    integer(c_int32_t), intent(in)   :: kind
+    integer(c_int64_t), intent(out)  :: nbound
+    integer(c_int32_t), allocatable, intent(out) :: bfaces(:,:)
+    integer(c_int64_t) :: e
+    integer :: i
+    allocate(bfaces(3, 2*mesh%n**2))
+    nbound = 0
+    do e = 1, mesh%nelem
+        i = nint(mesh%lower(1, e)*mesh%n)
+        if (i == 0) then
+            nbound = nbound + 1
+            bfaces(:, nbound) = (/ int(e-1, c_int32_t), 0_c_int32_t, kind /)
+            mesh%nbr(1, e) = -int(nbound, c_int32_t)
+        end if
+        if (i == mesh%n - 1) then
+            nbound = nbound + 1
+            bfaces(:, nbound) = (/ int(e-1, c_int32_t), 1_c_int32_t, kind /)
+            mesh%nbr(2, e) = -int(nbound, c_int32_t)
+        end if
+    end do
+end subroutine
+
+!! Sod's shock tube along x: (rho, p) = (1, 1) for x < 0.5 and (0.125, 0.1)
+!! beyond, at rest.
+subroutine fill_sod(mesh, nn, xi, kappa, state)
+    type(mesh_t), intent(in) :: mesh
+    integer, intent(in)      :: nn
+    real(dp), intent(in)     :: xi(nn), kappa
+    real(dp), intent(out)    :: state(nn, nn, nn, 5, mesh%nelem)
+    real(dp) :: x
+    integer(c_int64_t) :: e
+    integer :: i, j, k
+    do e = 1, mesh%nelem
+        do k = 1, nn; do j = 1, nn; do i = 1, nn
+            x = mesh%lower(1, e) + 0.5_dp*(xi(i) + 1.0_dp)*mesh%delta(1, e)
+            if (x < 0.5_dp) then
+                call prim2cons(kappa, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, state(i, j, k, :, e))
+            else
+                call prim2cons(kappa, 0.125_dp, 0.0_dp, 0.0_dp, 0.0_dp, 0.1_dp, state(i, j, k, :, e))
+            end if
+        end do; end do; end do
+    end do
+end subroutine
+
 !! prim2cons of GLM-MHD: equations/mhd/equations_mod.f90:294-310 (mu0 = 1;
 !! the energy holds B^2/2 and psi^2/2).
 pure subroutine prim2cons_mhd(kappa, rho, v, p, b, psi, u)

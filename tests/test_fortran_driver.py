@@ -1,8 +1,9 @@
 """The Fortran host driver (nemo_amd/fortran) over the C-ABI.
 
 GPU: the driver runs the reference's loop (calc_timestep, five stages per
-step, status check) through the ISO_C_BINDING interfaces for three schemes
-(Euler isentropic vortex, Euler Kelvin-Helmholtz, GLM-MHD Orszag-Tang);
+step, status check) through the ISO_C_BINDING interfaces for four setups
+(Euler isentropic vortex, Euler Kelvin-Helmholtz, GLM-MHD Orszag-Tang, and
+Sod's shock tube between slip walls);
 its dumped initial state is advanced by the oracle for the same number of
 steps and compared with the dumped final state (and the time steps).
 CPU: without a device the driver reports the C-ABI error and stops with a
@@ -44,7 +45,7 @@ def test_driver_reports_errors_without_device(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not _have_exe(), reason="Fortran driver not built")
-@pytest.mark.parametrize("setup", ["vortex", "kh", "ot"])
+@pytest.mark.parametrize("setup", ["vortex", "kh", "ot", "sod"])
 def test_fortran_driver_matches_oracle(tmp_path, setup):
     n, N, steps = 8, 3, 20
     prefix = str(tmp_path / setup)
@@ -58,11 +59,16 @@ def test_fortran_driver_matches_oracle(tmp_path, setup):
     dts = np.loadtxt(prefix + ".dt.txt")
     assert n_poly == N and len(dts) == steps
     m = meshmod.uniform_periodic(n)
+    if setup == "sod":
+        # slip walls across x, built here as the driver builds them
+        m = meshmod.with_boundaries(m, [meshmod.BC_SLIP_WALL] * 2 + [None] * 4)
     if setup == "vortex":
         cfg = O.make_cfg(N=N, volume=O.VOL_WEAK, surface=O.SURF_RUSANOV, timedisc=O.TD_SSPRK54)
-    elif setup == "kh":
+    elif setup in ("kh", "sod"):
         cfg = O.make_cfg(N=N, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
                          repair=1)
+        if setup == "sod":
+            cfg = O.with_boundaries(cfg, m.bfaces)
     else:
         # GLM-MHD: the oracle's run sets the cleaning speed and damping from
         # each step's CFL reduction, as the device does in calc_timestep

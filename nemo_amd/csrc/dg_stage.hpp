@@ -966,23 +966,20 @@ __global__ __launch_bounds__(BE * M * M) void cfl_kernel(const double* __restric
 // GLM-MHD also c_h = 2|lmax - vmax| and alpha = c_h/0.18
 // (kernel/mcfv/3d/kernel_timestep_mod.f90:53-56).
 // dt_dev: [0] dt, [1] lmax, [2] vmax, [3] c_h, [4] alpha.
-static __global__ __launch_bounds__(1024) void cfl_finalize(const double* __restrict__ part, long long npart, double cfl,
-                                                      int M, double* __restrict__ dt_dev, double* __restrict__ hist,
-                                                      long long slot) {
-    __shared__ double s[3][1024];
+// Two levels: block i of a grid of G <= CFL_GRID blocks reduces the i-th
+// contiguous G-th of the block partials (3 of the 4 values each: dt
+// candidate, lmax, vmax) into red[i]; the block that finishes last (ticket) reduces red and
+// writes dt_dev (and hist[slot]), then rearms the ticket for the next call.
+// min and max are exact, so the result does not depend on the order.
+constexpr int CFL_GRID = 1024, CFL_NT = 256;
+
+__device__ __forceinline__ void cfl_block_reduce(double& dt, double& lm, double& vm, double (*s)[CFL_NT]) {
     const int tid = threadIdx.x;
-    double dt = 1.7976931348623157e308, lm = 0.0, vm = 0.0;
-    for (long long i = tid; i < npart; i += 1024) {
-        const double* q = part + 4 * i;
-        dt = q[0] < dt ? q[0] : dt;
-        lm = q[1] > lm ? q[1] : lm;
-        vm = q[2] > vm ? q[2] : vm;
-    }
     s[0][tid] = dt;
     s[1][tid] = lm;
     s[2][tid] = vm;
     __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
+    for (int off = CFL_NT / 2; off > 0; off >>= 1) {
         if (tid < off) {
             s[0][tid] = s[0][tid + off] < s[0][tid] ? s[0][tid + off] : s[0][tid];
             s[1][tid] = s[1][tid + off] > s[1][tid] ? s[1][tid + off] : s[1][tid];
@@ -990,15 +987,58 @@ static __global__ __launch_bounds__(1024) void cfl_finalize(const double* __rest
         }
         __syncthreads();
     }
+    dt = s[0][0];
+    lm = s[1][0];
+    vm = s[2][0];
+}
+
+static __global__ __launch_bounds__(CFL_NT) void cfl_finalize(const double* __restrict__ part, long long npart,
+                                                              double cfl, int M, double* __restrict__ dt_dev,
+                                                              double* __restrict__ hist, long long slot,
+                                                              double* __restrict__ red, unsigned int* __restrict__ ticket) {
+    __shared__ double s[3][CFL_NT];
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    const long long G = gridDim.x;
+    const long long chunk = (npart + G - 1) / G, i0 = blockIdx.x * chunk;
+    const long long i1 = i0 + chunk < npart ? i0 + chunk : npart;
+    double dt = 1.7976931348623157e308, lm = 0.0, vm = 0.0;
+    for (long long i = i0 + tid; i < i1; i += CFL_NT) {
+        const double* q = part + 4 * i;
+        dt = q[0] < dt ? q[0] : dt;
+        lm = q[1] > lm ? q[1] : lm;
+        vm = q[2] > vm ? q[2] : vm;
+    }
+    cfl_block_reduce(dt, lm, vm, s);
     if (tid == 0) {
-        const double d = 0.5 * cfl / double(2 * M - 1) * s[0][0];
+        red[3 * blockIdx.x + 0] = dt;
+        red[3 * blockIdx.x + 1] = lm;
+        red[3 * blockIdx.x + 2] = vm;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == unsigned(G - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    dt = 1.7976931348623157e308;
+    lm = vm = 0.0;
+    for (long long i = tid; i < G; i += CFL_NT) {
+        const volatile double* q = red + 3 * i;
+        dt = q[0] < dt ? q[0] : dt;
+        lm = q[1] > lm ? q[1] : lm;
+        vm = q[2] > vm ? q[2] : vm;
+    }
+    cfl_block_reduce(dt, lm, vm, s);
+    if (tid == 0) {
+        const double d = 0.5 * cfl / double(2 * M - 1) * dt;
         dt_dev[0] = d;
-        dt_dev[1] = s[1][0];
-        dt_dev[2] = s[2][0];
-        const double ch = 2.0 * fabs(s[1][0] - s[2][0]);
+        dt_dev[1] = lm;
+        dt_dev[2] = vm;
+        const double ch = 2.0 * fabs(lm - vm);
         dt_dev[3] = ch;
         dt_dev[4] = ch / 0.18;
         if (hist && slot >= 0) hist[slot] = d;
+        *ticket = 0u;
     }
 }
 

@@ -138,8 +138,12 @@ void launch_cfl_m(Solver& s) {
 }
 
 int launch_finalize(Solver& s, long long hist_slot) {
-    hipLaunchKernelGGL(nemo::cfl_finalize, dim3(1), dim3(1024), 0, s.stream, s.part, nblocks_for(s), s.cfg.cfl, s.M,
-                       s.dt_dev, hist_slot >= 0 ? s.hist : nullptr, hist_slot);
+    const long long npart = nblocks_for(s);
+    long long G = (npart + nemo::CFL_NT - 1) / nemo::CFL_NT;
+    G = G < 1 ? 1 : (G > nemo::CFL_GRID ? nemo::CFL_GRID : G);
+    hipLaunchKernelGGL(nemo::cfl_finalize, dim3(unsigned(G)), dim3(nemo::CFL_NT), 0, s.stream, s.part, npart,
+                       s.cfg.cfl, s.M, s.dt_dev, hist_slot >= 0 ? s.hist : nullptr, hist_slot, s.cfl_red,
+                       s.cfl_ticket);
     HIP_TRY(hipGetLastError());
     return NEMO_OK;
 }
@@ -425,7 +429,9 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         hipMalloc(&s->mlp_m, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
         hipMalloc(&s->mlp_v, nemo::MLP_NP * sizeof(float)) != hipSuccess ||
         hipMalloc(&s->mlp_stats, 3 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&s->status, sizeof(int)) != hipSuccess) {
+        hipMalloc(&s->status, sizeof(int)) != hipSuccess ||
+        hipMalloc(&s->cfl_red, 3 * nemo::CFL_GRID * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s->cfl_ticket, sizeof(unsigned int)) != hipSuccess) {
         nemo_gpu_destroy(s);  // frees whichever of them were allocated
         return fail(NEMO_ERR_HIP, "hipMalloc failed");
     }
@@ -436,6 +442,7 @@ int nemo_gpu_create(const nemo_gpu_config* cfg, void** handle) {
         return fail(NEMO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     }
     (void)hipMemset(s->status, 0, sizeof(int));
+    (void)hipMemset(s->cfl_ticket, 0, sizeof(unsigned int));
     (void)hipMemset(s->dt_dev, 0, 8 * sizeof(double));
     (void)hipMemset(s->mlp_stats, 0, 3 * sizeof(unsigned long long));
     *handle = s;
@@ -459,6 +466,8 @@ int nemo_gpu_destroy(void* handle) {
     (void)hipFree(s->mlp_v);
     (void)hipFree(s->mlp_stats);
     (void)hipFree(s->hist);
+    (void)hipFree(s->cfl_red);
+    (void)hipFree(s->cfl_ticket);
     for (auto& p : s->ev) {
         (void)hipEventDestroy(p.first);
         (void)hipEventDestroy(p.second);

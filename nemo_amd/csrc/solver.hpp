@@ -43,7 +43,9 @@ struct Solver {
     double* an_part = nullptr;         // analysis partials
     long long an_cap = 0;
     double* an_out = nullptr;          // [NEMO_ANALYZE_NSUMS]
-    double* scratch = nullptr;         // this device's `scratch` (allocation-free analysis)
+    double* scratch = nullptr;
+    double* cfl_red = nullptr;         // [3 * CFL_GRID] second level of the CFL reduction
+    unsigned int* cfl_ticket = nullptr;  // blocks done in the current CFL reduction         // this device's `scratch` (allocation-free analysis)
     double* dt_dev = nullptr;  // [8]: dt, lmax, vmax, glm c_h, glm alpha
     double* part = nullptr;    // per-block CFL partials [nblocks][4]
     int* status = nullptr;

@@ -49,12 +49,12 @@ __device__ __forceinline__ void multi_block(const MultiStageArgs<Real>& ma, int 
     const int tid = threadIdx.x;
     if (g == 0) {
         if (tid < BatchTile<3>::BE * 9)
-            stage_block<EQ_EULER, 3, BatchTile<3>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME>(ma.a3, b, lds);
+            stage_block<EQ_EULER, 3, BatchTile<3>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME, false>(ma.a3, b, lds);
     } else if (g == 1) {
-        stage_block<EQ_EULER, 4, BatchTile<4>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME>(ma.a4, b, lds);
+        stage_block<EQ_EULER, 4, BatchTile<4>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME, false>(ma.a4, b, lds);
     } else {
         if (tid < BatchTile<5>::BE * 25)
-            stage_block<EQ_EULER, 5, BatchTile<5>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME>(ma.a5, b, lds);
+            stage_block<EQ_EULER, 5, BatchTile<5>::BE, VOL_EC, SURF_HLLC, Real, IK_RUNTIME, false>(ma.a5, b, lds);
     }
 }
 

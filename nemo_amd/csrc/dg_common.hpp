@@ -161,6 +161,12 @@ __device__ __forceinline__ long long xcd_block(long long nblocks) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// A compile-time flag passed as an argument (selects a variant of a pass).
+template <bool B>
+struct BoolConst {
+    static constexpr bool value = B;
+};
+
 template <int M, int BE>
 struct StageShape {
     static constexpr int M2 = M * M, M3 = M2 * M;

@@ -198,8 +198,10 @@ struct XUpdate {
 // store its contributions: it adds them to the accumulated surface and y/z
 // terms and applies the LSRK probe to its own line's nodes (register out to
 // HBM, new state into xu), with the same arithmetic as the separate update.
-// Without fuse it stores them like the other passes.
-template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU, bool XUPD = false>
+// Without fuse it stores them like the other passes.  NEAR1: the caller has
+// found every ratio of two densities and of two betas of the block in
+// [sqrt(1/2), sqrt(2)) (ec_flux).
+template <int D, int M, int BE, int VOL, int SURF, typename Real, typename TU, bool XUPD = false, bool NEAR1 = false>
 __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long long e0, const TU (*s_u)[BE * M * M * M],
                                                Real (*s_acc)[BE * M * M * M], const Real alpha,
                                                const double* s_p64, const ElemGeom& geo,
@@ -270,7 +272,7 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
             for (int m = i + 1; m < M; ++m) {
                 Real F[5];
                 if constexpr (VOL == VOL_EC)
-                    ec_flux<D>(g, S[i], S[m], F);
+                    ec_flux<D, NEAR1>(g, S[i], S[m], F);
                 else
                     central_flux<D>(S[i], S[m], F);
 #pragma unroll

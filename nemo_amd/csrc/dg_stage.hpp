@@ -539,7 +539,7 @@ __device__ __forceinline__ double probe_lsrk(const StageArgs<M, Real>& a, long l
 
 // One block of BE elements through a whole stage (the body of the stage
 // kernels); lds holds StageLds::DOUBLES doubles.
-template <int EQ, int M, int BE, int VOL, int SURF, typename Real, int IK>
+template <int EQ, int M, int BE, int VOL, int SURF, typename Real, int IK, bool NEAR1_SPLIT = true>
 __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const long long blk, double* lds) {
     using Sh = StageShape<M, BE>;
     using L = StageLds<EQ, M, BE, Real>;
@@ -577,6 +577,12 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     // HBM once, not once per node and variable.
     static_assert(NN % NT == 0, "whole lines per thread");
     constexpr int QS = NN / NT;
+    // FP64 two-point volume flux (Euler): whether the log-means may skip the
+    // log's range reduction (NEAR1 of the direction passes), decided for the
+    // whole block during staging so that each variant of the passes is
+    // straight-line code.
+    constexpr bool NEAR1_TEST = NEAR1_SPLIT && EQ == EQ_EULER && !F32 && VOL == VOL_EC;
+    bool near1 = false;
     FaceNodes fn;
     {
         double uu[V][QS];
@@ -608,20 +614,49 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 s_p64[sl] = pressure64<EQ>(a.ph64, un);
             }
         }
+        if constexpr (NEAR1_TEST) {
+            // Every node's density and pressure within a factor F of node 0's
+            // (both positive): two densities of the block are then within
+            // F^2 = 1.18 of each other and two betas rho/(2p) within F^4 =
+            // 1.386 (1.4 with the few ulps by which the passes' pressures,
+            // summed in their own order, differ from this one), so every
+            // ratio the log-means take lies in [sqrt(1/2), sqrt(2)).  NaN
+            // or non-positive states fail the test.
+            constexpr double F = 1.085;
+            const double rref = __shfl(uu[DENS][0], 0);
+            const double eref = __shfl(uu[ENER][0], 0);
+            const double kref = __shfl(uu[MOMX][0] * uu[MOMX][0] + uu[MOMY][0] * uu[MOMY][0] +
+                                       uu[MOMZ][0] * uu[MOMZ][0], 0);
+            const double km1 = a.ph64.gas.kappa_m1;
+            const double pref = km1 * (eref - 0.5 * kref * rcp_nr(rref));
+            bool ok = rref >= 1e-300 && pref >= 1e-300;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const double r = uu[DENS][q];
+                const double k = uu[MOMX][q] * uu[MOMX][q] + uu[MOMY][q] * uu[MOMY][q] + uu[MOMZ][q] * uu[MOMZ][q];
+                const double p = km1 * (uu[ENER][q] - 0.5 * k * rcp_nr(r));
+                ok = ok && r <= F * rref && rref <= F * r && p <= F * pref && pref <= F * p;
+            }
+            // readfirstlane: a wavefront-uniform value the compiler can see
+            // as such, so that the passes' two variants form a scalar branch
+            // rather than two exec-masked regions (whose values would all be
+            // live at once)
+            near1 = __builtin_amdgcn_readfirstlane(int(__all(ok))) != 0;
+        }
     }
-    // FP64 Euler with the two-point volume flux, M <= 4: the LSRK register
-    // of the RK update is requested here, so that it arrives during the
-    // operator instead of being waited for at the end.  Its 40 VGPRs fit
-    // under the 256 of two waves per SIMD there (not for the other volume
-    // forms or M = 5).  Unconditional loads -- stage 1 and the SSP scheme
-    // read u_in instead and ignore it -- keep the wait counts exact.
-    // The x pass then runs last and, with LSRK, applies the update to its
-    // own line's nodes (XUPD), so the register is requested in that layout:
-    // the M nodes of the thread's x line (QS = M).
+    // FP64 Euler with the two-point volume flux, M <= 4, blocks that pass the
+    // near-1 test: the LSRK register of the RK update is requested before
+    // the direction passes (load_r0), so that it arrives during the operator
+    // instead of being waited for at the end; its 40 VGPRs fit under the 256
+    // of two waves per SIMD there (not in the general variant of the passes,
+    // nor for the other volume forms or M = 5).  Unconditional loads --
+    // stage 1 and the SSP scheme read u_in instead and ignore it -- keep the
+    // wait counts exact.  The x pass then runs last and, with LSRK, applies
+    // the update to its own line's nodes (XUPD), so the register is
+    // requested in that layout: the M nodes of the thread's x line (QS = M).
     constexpr bool R0_EARLY = EQ == EQ_EULER && !F32 && VOL == VOL_EC && M <= 4;
     const bool xlast = R0_EARLY && a.timedisc == TD_LSRK54 && !a.rhs_out;
-    double r0e[R0_EARLY ? V : 1][R0_EARLY ? QS : 1];
-    if constexpr (R0_EARLY) {
+    auto load_r0 = [&](double (&r0e)[V][QS]) {
         static_assert(QS == M, "one x line per thread");
         const double* rsrc = (a.timedisc == TD_LSRK54 && a.stage != 1) ? a.r0 : a.u_in;
         const int lel = tid / Sh::M2, lline = tid - lel * Sh::M2;
@@ -632,7 +667,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
             for (int q = 0; q < QS; ++q) {
                 r0e[h][q] = rsrc[h * EM3 + ex * M3 + nidx<0, M>(q, lline % M, lline / M)];
             }
-    }
+    };
     __syncthreads();
 
     Real alpha = Real(0);
@@ -660,20 +695,33 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         surface_pass<2, M, BE, SURF, Real, TU>(a, e0, s_u, s_acc, s_p64, geo, fn);
         __syncthreads();
         if constexpr (R0_EARLY) {
-            // x runs last (one code path for both tails: with LSRK it fuses
-            // the update, otherwise it accumulates like the others)
-            direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-            __syncthreads();
-            direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-            __syncthreads();
+            // x runs last (in the near-1 variant with LSRK it fuses the
+            // update, otherwise it accumulates like the others)
             XUpdate<M> xu;
             xu.dt = a.dt_run ? a.dt_run[a.block_run[blk]] : (a.dt_dev ? *a.dt_dev : a.dt);
-#pragma unroll
-            for (int h = 0; h < V; ++h)
-#pragma unroll
-                for (int q = 0; q < QS; ++q) xu.r0[h][q] = r0e[h][q];
-            direction_pass<0, M, BE, VOL, SURF, Real, TU, true>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu, xlast);
-            if (xlast) {
+            // The near-1 variant fuses the update into the x pass and ends in
+            // its own store of the new state (to HBM, or to LDS for the tail
+            // below); the general variant (blocks with steep gradients)
+            // accumulates x like the other passes and leaves the update to
+            // the tail, so that no register is live across its passes.
+            auto passes = [&](auto near1_tag) -> bool {
+                constexpr bool NR = decltype(near1_tag)::value;
+                // distinct empty markers keep the compiler from hoisting the
+                // variants' common first instructions above the fork, where
+                // their registers would be live in both variants (it spilled
+                // over a hundred VGPRs without them)
+                if constexpr (NR)
+                    asm volatile("; near1 passes");
+                else
+                    asm volatile("; general passes");
+                if constexpr (NR) load_r0(xu.r0);
+                direction_pass<1, M, BE, VOL, SURF, Real, TU, false, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                direction_pass<2, M, BE, VOL, SURF, Real, TU, false, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                direction_pass<0, M, BE, VOL, SURF, Real, TU, NR, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo, &xu,
+                                                                      NR && xlast);
+                if (!NR || !xlast) return false;
                 // The new state of the thread's x line: stored straight to
                 // HBM unless a repair or the CFL partials need the block in
                 // LDS (positivity test as in the repair below).
@@ -700,20 +748,34 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 #pragma unroll
                             for (int q = 0; q < QS; ++q) a.u_out[h * EM3 + eb + nidx<0, M>(q, lta, ltb)] = xu.unew[h][q];
                     }
-                    return;
+                    return true;
                 }
                 __syncthreads();
 #pragma unroll
                 for (int h = 0; h < V; ++h)
 #pragma unroll
                     for (int q = 0; q < QS; ++q) s_u64[h][bslot<M>(lel * M3, nidx<0, M>(q, lta, ltb))] = xu.unew[h][q];
-            }
+                return false;
+            };
+            const bool done = near1 ? passes(BoolConst<true>{}) : passes(BoolConst<false>{});
+            if (done) return;
         } else {
-            direction_pass<0, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-            __syncthreads();
-            direction_pass<1, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
-            __syncthreads();
-            direction_pass<2, M, BE, VOL, SURF, Real, TU>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            auto passes = [&](auto near1_tag) {
+                constexpr bool NR = decltype(near1_tag)::value;
+                if constexpr (NR)
+                    asm volatile("; near1 passes");
+                else
+                    asm volatile("; general passes");
+                direction_pass<0, M, BE, VOL, SURF, Real, TU, false, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                direction_pass<1, M, BE, VOL, SURF, Real, TU, false, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+                __syncthreads();
+                direction_pass<2, M, BE, VOL, SURF, Real, TU, false, NR>(a, e0, s_u, s_acc, alpha, s_p64, geo);
+            };
+            if (NEAR1_TEST && near1)
+                passes(BoolConst<NEAR1_TEST>{});
+            else
+                passes(BoolConst<false>{});
         }
     }
     __syncthreads();
@@ -820,7 +882,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 const int idx = tid + q * NT;
                 if (idx < nvalid_nodes) s_u64[h][slot<M>(idx)] = st[h][q];
             }
-    } else if (xlast) {
+    } else if (xlast && near1) {
         // done by the x pass (XUPD)
     } else if (lsrk) {
 #pragma unroll
@@ -897,10 +959,12 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
 // FP64 Euler at N=4: left to itself the compiler takes 256 VGPRs and a few
 // AGPRs, one wave per SIMD, although the 20 KB of LDS allow two; asked for
 // two it fits in at most 245 VGPRs without spills and runs 1.5x faster.
+// At N=3 the request keeps the two paths of the log-means (with and without
+// the log's range reduction) within 256 VGPRs as well.
 // (The same request for three waves at N=2 spills and runs slower.)
 template <int EQ, int M, typename Real>
 constexpr int stage_waves_per_eu() {
-    return (EQ == EQ_EULER && M == 5 && sizeof(Real) == 8) ? 2 : 1;
+    return (EQ == EQ_EULER && M >= 4 && sizeof(Real) == 8) ? 2 : 1;
 }
 
 // One workgroup per block of BE elements.
@@ -934,7 +998,7 @@ __global__ __launch_bounds__(BE * M * M) void stage_kernel_persistent(StageArgs<
             b = it * G + blockIdx.x;
         }
         if (b >= nblocks) continue;
-        stage_block<EQ, M, BE, VOL, SURF, Real, IK_RUNTIME>(a, b, lds);
+        stage_block<EQ, M, BE, VOL, SURF, Real, IK_RUNTIME, false>(a, b, lds);
         __syncthreads();
     }
 }

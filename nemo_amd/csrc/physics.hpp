@@ -103,17 +103,8 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(
 // near 1, where the log-mean divides by it.  The three-argument form also
 // returns t = s^2; with unscaled set it takes m = x, k = 0 (then t is
 // ((x-1)/(x+1))^2 for any x, and the log is valid only for x near 1).
-__device__ __forceinline__ double log_pos(double x, bool unscaled, double& t_out) {
-    int k = __builtin_amdgcn_frexp_exp(x);
-    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
-    if (m < 0.70710678118654752440) {
-        m = m + m;
-        k = k - 1;
-    }
-    if (unscaled) {
-        m = x;
-        k = 0;
-    }
+// log_series is the reduced part, log m for m in [sqrt(1/2), sqrt(2)), with t.
+__device__ __forceinline__ double log_series(double m, double& t_out) {
     const double f = m - 1.0;
     const double s = f * rcp_nr(m + 1.0);
     const double t = s * s;
@@ -134,8 +125,21 @@ __device__ __forceinline__ double log_pos(double x, bool unscaled, double& t_out
     P = fma_sc(P, t, 1.0 / 7.0);
     P = fma_sc(P, t, 1.0 / 5.0);
     P = fma_sc(P, t, 1.0 / 3.0);
-    const double lm = fma(2.0 * s * t, P, 2.0 * s);
     t_out = t;
+    return fma(2.0 * s * t, P, 2.0 * s);
+}
+__device__ __forceinline__ double log_pos(double x, bool unscaled, double& t_out) {
+    int k = __builtin_amdgcn_frexp_exp(x);
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m = m + m;
+        k = k - 1;
+    }
+    if (unscaled) {
+        m = x;
+        k = 0;
+    }
+    const double lm = log_series(m, t_out);
     return fma(double(k), 0.69314718055994530942, lm);
 }
 __device__ __forceinline__ double log_pos(double x) {
@@ -329,7 +333,11 @@ __device__ __forceinline__ void riemann_hllc(const Gas<Real>& g, const NodeState
 // logmean, equations/euler/two_point/chandrashekar/two_point_flux_mod.f90:
 // 13-32 (series branch as written, including the sign of its u^2 term),
 // returned together with its reciprocal so that the flux needs no further
-// division by it: 1/L = log(a/b)/(a-b) on the log branch.
+// division by it: 1/L = log(a/b)/(a-b) on the log branch.  NEAR1: the
+// caller guarantees a/b in [sqrt(1/2), sqrt(2)) after rounding, where the
+// log's range reduction leaves m = a/b and k = 0; it is skipped, the
+// remaining operations are the same, and so are the bits.
+template <bool NEAR1 = false>
 __device__ __forceinline__ void logmean2(double a, double b, double& lm, double& inv_lm) {
     // Both branches are evaluated and selected: straight-line code lets the
     // scheduler interleave the independent log-means of a line, where
@@ -344,7 +352,7 @@ __device__ __forceinline__ void logmean2(double a, double b, double& lm, double&
     // O(1), u < 1e-8 and enters only as u/6 against 1/2: the difference
     // between the two roundings of it is below 1e-16 of the log-mean.
     double u;
-    const double l = log_pos(x, series, u);
+    const double l = NEAR1 ? log_series(x, u) : log_pos(x, series, u);
     double w;
     {
 #pragma clang fp contract(off)
@@ -365,7 +373,8 @@ __device__ __forceinline__ void logmean2(double a, double b, double& lm, double&
 // lane of the wavefront needs it (a wavefront-uniform branch): in smooth
 // flow a pair then costs one reciprocal (two with the mean itself) instead
 // of five transcendental instructions, which issue at a quarter of the
-// VALU rate.
+// VALU rate.  (NEAR1 has no use here: the series covers those ratios.)
+template <bool NEAR1 = false>
 __device__ __forceinline__ void logmean2(float a, float b, float& lm, float& inv_lm) {
     const float s = a + b;
     const float r = __builtin_amdgcn_rcpf(s);
@@ -408,14 +417,15 @@ __device__ __forceinline__ void logmean2_select(float a, float b, float& lm, flo
 }
 
 // Chandrashekar entropy-conserving, kinetic-energy-preserving flux:
-// two_point_flux_mod.f90:34-77, rotated to direction D (:79-141).
-template <int D, typename Real>
+// two_point_flux_mod.f90:34-77, rotated to direction D (:79-141).  NEAR1 as
+// for logmean2, for the ratios of both densities and both betas.
+template <int D, bool NEAR1 = false, typename Real>
 __device__ __forceinline__ void ec_flux(const Gas<Real>& g, const NodeState<Real>& a, const NodeState<Real>& b,
                                         Real* f) {
     constexpr int N0 = D, N1 = (D + 1) % 3, N2 = (D + 2) % 3;
     Real dens, inv_dens, lnb, inv_lnb;
-    logmean2(a.u[DENS], b.u[DENS], dens, inv_dens);
-    logmean2(a.beta, b.beta, lnb, inv_lnb);
+    logmean2<NEAR1>(a.u[DENS], b.u[DENS], dens, inv_dens);
+    logmean2<NEAR1>(a.beta, b.beta, lnb, inv_lnb);
     const Real velx = Real(0.5) * (a.v[N0] + b.v[N0]);
     const Real vely = Real(0.5) * (a.v[N1] + b.v[N1]);
     const Real velz = Real(0.5) * (a.v[N2] + b.v[N2]);

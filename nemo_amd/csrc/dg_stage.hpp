@@ -834,7 +834,12 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
     const bool lsrk = a.timedisc == TD_LSRK54;
     if constexpr (COMPACT) {
         // FP64 state from HBM, new state in registers until the operator's
-        // LDS is free.  With LSRK the FP32 operator keeps its register in
+        // LDS is free.  The register and the new state stream through L2
+        // as nontemporal accesses, which it evicts first, so that more of
+        // the state the update re-reads here is still in L2 (FP32 + PP
+        // stage 3.6 % faster; the FP64 kernel, whose fused x pass stores
+        // 32-byte pieces per instruction, ran 11 % slower with them).
+        // With LSRK the FP32 operator keeps its register in
         // FP32 (the r0 buffer read as floats, half the register traffic):
         // reg = a_s r0 + f is formed in FP64 and rounded once, and the state
         // advances by the rounded value, so the next stage continues from
@@ -852,7 +857,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                 const bool ok = idx < nvalid_nodes;
                 uin[q] = ok ? a.u_in[gi] : 0.0;
                 if (lsrk)
-                    reg32[q] = (ok && s != 1) ? r0f[gi] : 0.0f;
+                    reg32[q] = (ok && s != 1) ? __builtin_nontemporal_load(&r0f[gi]) : 0.0f;
                 else
                     reg[q] = ok ? a.r0[gi] : 0.0;
             }
@@ -866,7 +871,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                     if (lsrk) {
                         // low-storage/4-5/timedisc_probe_mod.f90:31-37
                         const float rg = float(s == 1 ? f : a.rk.lsrk_a[s - 1] * double(reg32[q]) + f);
-                        r0f[gi] = rg;
+                        __builtin_nontemporal_store(rg, &r0f[gi]);
                         st[h][q] = uin[q] + a.rk.lsrk_c[s - 1] * dt * double(rg);
                     } else {
                         st[h][q] = probe(gi, uin[q], f);
@@ -944,7 +949,10 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         for (int idx = tid; idx < nvalid_nodes; idx += NT) {
             const long long gi = h * EM3 + e0 * M3 + idx;
             const double st = s_u64[h][slot<M>(idx)];
-            a.u_out[gi] = st;
+            if constexpr (COMPACT)
+                __builtin_nontemporal_store(st, &a.u_out[gi]);
+            else
+                a.u_out[gi] = st;
             if (a.timedisc == TD_SSPRK54) {
                 if (s == 2) a.r1[gi] = a.rk.ssp[20] * st;
                 else if (s == 3) a.r1[gi] = a.r1[gi] + a.rk.ssp[21] * st;

@@ -103,6 +103,7 @@ class Forest:
     origin: tuple = (0.0, 0.0, 0.0)
     _conn: tuple = field(default=None, repr=False, compare=False)
     _fam: np.ndarray = field(default=None, repr=False, compare=False)
+    _cols: tuple = field(default=None, repr=False, compare=False)
 
     # ---- construction ----------------------------------------------------
     @classmethod
@@ -214,6 +215,18 @@ class Forest:
         ii, jj, kk = np.meshgrid(np.arange(M), np.arange(M), np.arange(M), indexing="ij")
         ref = np.stack([xi[ii].ravel(order="F"), xi[jj].ravel(order="F"), xi[kk].ravel(order="F")])
         return self.lower[:, :, None] + 0.5 * (ref[None, :, :] + 1.0) * self.delta[:, :, None]
+
+    def neighbour_columns(self):
+        """The conforming face neighbours as six index arrays, one per face,
+        holding n (a slot past the last leaf) where the face has none, and the
+        big sides [nm] and small sides [nm, 4] of the mortars; cached with the
+        forest (the gathers of amr_run.face_neighbours_any)."""
+        if self._cols is None:
+            nbr, mortars = self.connectivity()
+            n = self.nelem
+            cols = tuple(np.ascontiguousarray(np.where(nbr[:, f] >= 0, nbr[:, f], n)) for f in range(6))
+            self._cols = (cols, np.ascontiguousarray(mortars[:, 0]), np.ascontiguousarray(mortars[:, 2:]))
+        return self._cols
 
     def connectivity(self):
         """(nbr [n, 6] int32, mortars [nm, 6] int32) of the upload format."""

@@ -23,16 +23,16 @@ from . import amr
 
 def face_neighbours_any(forest: amr.Forest, flag: np.ndarray) -> np.ndarray:
     """flag | (flag of any face neighbour), across conforming and 2:1 faces."""
-    nbr, mortars = forest.connectivity()
+    cols, big, small = forest.neighbour_columns()
     flag = np.asarray(flag, dtype=bool)
-    n = flag.shape[0]
-    conf = nbr >= 0
-    out = flag | (conf & flag[np.where(conf, nbr, 0)]).any(axis=1)
-    if len(mortars):
-        big, small = mortars[:, 0], mortars[:, 2:]
-        # a leaf may sit on several mortars: count the flagged ones
-        out |= np.bincount(big, weights=flag[small].any(axis=1), minlength=n) > 0
-        out |= np.bincount(small.ravel(), weights=np.repeat(flag[big], 4), minlength=n) > 0
+    ext = np.append(flag, False)     # the slot of a face without a conforming neighbour
+    out = flag.copy()
+    for c in cols:
+        out |= ext[c]
+    if len(big):
+        # a leaf may sit on several mortars: any flagged side marks it
+        out[big[flag[small].any(axis=1)]] = True
+        out[small[flag[big]].ravel()] = True
     return out
 
 
@@ -83,23 +83,43 @@ class AdaptiveRun:
     # the flags, and so the outcome, are a function of the forest and these
     # masks, so an equal pair on the same forest is steady again
     _steady: tuple = field(default=None, repr=False)
+    # the flags of that pass as Forest.adapt sees them (refine below the
+    # finest level, coarsen): its outcome is a function of the forest and
+    # these, so equal flags on the same forest leave it unchanged again --
+    # the masks move from pass to pass where a factor hovers at the
+    # threshold while the flags after the neighbour layer stay put
+    _steady_flags: tuple = field(default=None, repr=False)
+    steady_shortcuts: bool = True   # False: run every pass in full (tests)
+    mask_shortcuts: int = 0         # passes decided by the mask test
+    flag_shortcuts: int = 0         # passes decided by the flag test
 
     def adapt(self) -> np.ndarray | None:
         t0 = time.perf_counter()
         alpha = self.solver.indicator()
         masks = (alpha > self.refine_above, alpha <= self.coarsen_below)
-        if self._steady is not None and all(np.array_equal(m, k) for m, k in zip(masks, self._steady)):
+        if self.steady_shortcuts and self._steady is not None and \
+                all(np.array_equal(m, k) for m, k in zip(masks, self._steady)):
+            self.mask_shortcuts += 1
             self.remesh_seconds += time.perf_counter() - t0
             return None
         refine, coarsen = adapt_flags(self.forest, alpha, self.refine_above, self.coarsen_below)
+        flags = (refine & (self.forest.level < self.forest.maxlevel), coarsen)
+        if self.steady_shortcuts and self._steady_flags is not None and \
+                all(np.array_equal(m, k) for m, k in zip(flags, self._steady_flags)):
+            self._steady = masks
+            self.flag_shortcuts += 1
+            self.remesh_seconds += time.perf_counter() - t0
+            return None
         new, units = self.forest.adapt(refine, coarsen)
         if units is not None:
             self.fallbacks += self.solver.remesh(new, units)
             self.forest = new
             self.remeshes += 1
             self._steady = None
+            self._steady_flags = None
         else:
             self._steady = masks
+            self._steady_flags = flags
         self.remesh_seconds += time.perf_counter() - t0
         return units
 

@@ -184,6 +184,67 @@ def test_face_neighbour_flags_by_geometry():
         assert np.array_equal(amr_run.face_neighbours_any(f, flag), want)
 
 
+class _OracleSolver:
+    """The calls amr_run.AdaptiveRun makes of a solver, answered by the
+    oracle (indicator, CPU steps with the forest's mortars, state transfer)."""
+    M3 = 64
+
+    def __init__(self, cfg, forest, u):
+        self.cfg, self.forest, self.u = cfg, forest, np.ascontiguousarray(u)
+
+    def indicator(self):
+        return O.indicator(self.cfg, self.u)[0]
+
+    def run(self, k, record_dt=False):
+        rc, h = O.run(O.with_mortars(self.cfg, self.forest.mortars), self.forest.nbr, self.forest.delta, self.u, k)
+        assert rc == 0
+        return h
+
+    def remesh(self, new, units):
+        u, nfb = O.remap(3, self.cfg.gamma, self.u, units, new.nelem)
+        self.u, self.forest = np.ascontiguousarray(u), new
+        return nfb
+
+
+def test_adaptive_run_steady_shortcuts_are_exact():
+    """AdaptiveRun skips a pass whose indicator masks, or whose refine and
+    coarsen flags (as Forest.adapt reads them), equal those of the last pass
+    that left the same forest unchanged.  A Sedov blast on levels 2-4
+    (oracle steps, adaptation every 8 of 96 steps) gives the same meshes,
+    replacement units, time steps and final state with the shortcuts as
+    with every pass run in full, and the flag test decides passes whose
+    masks moved."""
+    from nemo_amd import amr_run
+    cfg = O.make_cfg(N=3, volume=O.VOL_EC, surface=O.SURF_HLLC, indicator=O.IND_PP, timedisc=O.TD_LSRK54,
+                     repair=1, cfl=0.5)
+    ic = lambda f: indicator_mlp.sedov(f, 3, seed=1234, radius=0.08, p0=1e-5)
+    f = amr.Forest.uniform(2, minlevel=2, maxlevel=4)
+    for _ in range(8):
+        a = O.indicator(cfg, np.ascontiguousarray(ic(f)))[0]
+        new, units = f.adapt(amr_run.face_neighbours_any(f, a > 0), np.zeros(f.nelem, bool))
+        if units is None:
+            break
+        f = new
+    runs = {}
+    for sc in (True, False):
+        s = _OracleSolver(cfg, f, ic(f))
+        run = amr_run.AdaptiveRun(s, f, every=8, steady_shortcuts=sc)
+        run.advance(96, record=True)
+        runs[sc] = (run, s.u)
+    (a, ua), (b, ub) = runs[True], runs[False]
+    assert a.flag_shortcuts > 0
+    assert b.flag_shortcuts == b.mask_shortcuts == 0
+    assert a.remeshes == b.remeshes >= 1
+    assert len(a.history) == len(b.history) == 12
+    for x, y in zip(a.history, b.history):
+        assert np.array_equal(x.forest.level, y.forest.level) and np.array_equal(x.forest.anchor, y.forest.anchor)
+        assert (x.units is None) == (y.units is None)
+        if x.units is not None:
+            assert np.array_equal(x.units, y.units)
+        assert np.array_equal(x.dts, y.dts)
+    assert np.array_equal(ua, ub)
+
+
 def test_adapt_refine_coarsen_inverse():
     f = _forest(lmax=5)
     flags = np.zeros(f.nelem, dtype=bool)

@@ -49,23 +49,51 @@ UNIT_COPY, UNIT_REFINE, UNIT_COARSEN = 0, 1, 2
 TANGENTIAL = ((1, 2), (0, 2), (0, 1))
 
 
-def _keys(level: np.ndarray, anchor: np.ndarray) -> np.ndarray:
-    """Lookup key of (level, anchor): Morton code of the anchor, then level."""
-    m = morton_key(anchor[:, 0], anchor[:, 1], anchor[:, 2]).astype(np.int64)
-    return m * 32 + level.astype(np.int64)
+def _mkeys(anchor: np.ndarray) -> np.ndarray:
+    """Morton codes (uint64) of lattice anchors [n, 3]."""
+    return morton_key(anchor[:, 0], anchor[:, 1], anchor[:, 2])
+
+
+# Moves on the Morton code itself (x in bits 0, 3, 6, ..., y in 1, 4, ...,
+# z in 2, 5, ...).  Each axis is a dilated integer: with M its bit mask,
+# x + y = ((x | ~M) + y) & M (the carry runs through the other axes' bits,
+# set to 1) and x - y = ((x & M) - y) & M; bits past the lattice drop out,
+# which is the periodic wrap of the box.
+_AXIS_MASK = [np.uint64(sum(1 << (3 * i + d) for i in range(GRID_BITS))) for d in range(3)]
+
+
+def _dilated(size: np.ndarray, d: int) -> np.ndarray:
+    """A power-of-two lattice length along axis d, in Morton-code bits."""
+    e = np.log2(size.astype(np.float64)).astype(np.uint64)
+    return np.left_shift(np.uint64(1), np.uint64(3) * e + np.uint64(d))
+
+
+def _key_move(key: np.ndarray, d: int, step: np.ndarray, s: int) -> np.ndarray:
+    """Morton codes of the anchors moved by s * (length) along axis d
+    (periodic), step = that length from _dilated."""
+    M = _AXIS_MASK[d]
+    moved = (((key | ~M) + step) if s > 0 else ((key & M) - step)) & M
+    return (key & ~M) | moved
+
+
+def _key_floor(key: np.ndarray, size: np.ndarray) -> np.ndarray:
+    """Morton codes of the anchors rounded down to multiples of size (a power
+    of two, per anchor) in every axis."""
+    e = np.log2(size.astype(np.float64)).astype(np.uint64)
+    return key & ~(np.left_shift(np.uint64(1), np.uint64(3) * e) - np.uint64(1))
 
 
 class _Index:
-    """Sorted-key lookup of leaves by (level, anchor)."""
+    """Sorted-key lookup of leaves by (level, Morton code of the anchor)."""
 
-    def __init__(self, level: np.ndarray, anchor: np.ndarray):
-        k = _keys(level, anchor)
+    def __init__(self, level: np.ndarray, mkey: np.ndarray):
+        k = mkey.astype(np.int64) * 32 + level.astype(np.int64)
         self.order = np.argsort(k, kind="stable")
         self.sorted = k[self.order]
 
-    def find(self, level: np.ndarray, anchor: np.ndarray) -> np.ndarray:
-        """Leaf index of each (level, anchor), or -1."""
-        k = _keys(level, anchor)
+    def find(self, level: np.ndarray, mkey: np.ndarray) -> np.ndarray:
+        """Leaf index of each (level, Morton code of the anchor), or -1."""
+        k = mkey.astype(np.int64) * 32 + level.astype(np.int64)
         pos = np.searchsorted(self.sorted, k)
         pos = np.minimum(pos, len(self.sorted) - 1)
         hit = self.sorted[pos] == k
@@ -183,14 +211,14 @@ class Forest:
         point's key."""
         f = self
         while True:
-            keys = morton_key(f.anchor[:, 0], f.anchor[:, 1], f.anchor[:, 2]).astype(np.int64)
+            mk = _mkeys(f.anchor)
+            keys = mk.astype(np.int64)
             h = _size(f.level)
             need = np.zeros(f.nelem, dtype=bool)
             for d in range(3):
+                step = _dilated(h, d)
                 for s in (-1, 1):
-                    p = f.anchor.copy()
-                    p[:, d] = (p[:, d] + s * h) % ROOT
-                    kp = morton_key(p[:, 0], p[:, 1], p[:, 2]).astype(np.int64)
+                    kp = _key_move(mk, d, step, s).astype(np.int64)
                     c = np.searchsorted(keys, kp, side="right") - 1
                     bad = f.level[c] < f.level - 1
                     need[c[bad]] = True
@@ -233,7 +261,8 @@ class Forest:
         if self._conn is not None:
             return self._conn
         n = self.nelem
-        index = _Index(self.level, self.anchor)
+        mk = _mkeys(self.anchor)
+        index = _Index(self.level, mk)
         h = _size(self.level)
         nbr = np.full((n, 6), np.iinfo(np.int32).min, dtype=np.int64)
         big_faces = []  # (element, face, children[4])
@@ -242,29 +271,30 @@ class Forest:
             for side in (0, 1):
                 f = 2 * d + side
                 s = 1 if side else -1
-                p = self.anchor.copy()
-                p[:, d] = (p[:, d] + s * h) % ROOT
-                same = index.find(self.level, p)
+                kp = _key_move(mk, d, _dilated(h, d), s)   # anchor of the same-size neighbour
+                same = index.find(self.level, kp)
                 nbr[same >= 0, f] = same[same >= 0]
                 rest = np.nonzero(same < 0)[0]
                 if len(rest) == 0:
                     continue
                 # coarser neighbour: this element is a small side (filled from the big side below)
                 lvl = self.level[rest]
-                pm = p[rest] & ~(2 * h[rest] - 1)[:, None]
-                coarse = index.find(lvl - 1, pm)
+                coarse = index.find(lvl - 1, _key_floor(kp[rest], 2 * h[rest]))
                 fine_idx = rest[coarse < 0]
                 if len(fine_idx) == 0:
                     continue
-                # finer neighbours: this element is the big side of a mortar
+                # finer neighbours: this element is the big side of a mortar;
+                # the same-size anchor is a multiple of h, so adding half of h
+                # along an axis sets one bit
                 hh = h[fine_idx] // 2
+                base = kp[fine_idx] | (_dilated(hh, d) if side == 0 else np.uint64(0))
                 kids = np.empty((len(fine_idx), 4), dtype=np.int64)
                 for k in range(4):
-                    a = p[fine_idx].copy()
-                    if side == 0:
-                        a[:, d] += hh
-                    a[:, t1] += (k & 1) * hh
-                    a[:, t2] += (k >> 1) * hh
+                    a = base.copy()
+                    if k & 1:
+                        a |= _dilated(hh, t1)
+                    if k >> 1:
+                        a |= _dilated(hh, t2)
                     kids[:, k] = index.find(self.level[fine_idx] + 1, a)
                 if (kids < 0).any():
                     raise ValueError("mesh is not 2:1 balanced across faces")

@@ -184,6 +184,28 @@ def test_face_neighbour_flags_by_geometry():
         assert np.array_equal(amr_run.face_neighbours_any(f, flag), want)
 
 
+def test_morton_code_moves():
+    """Moves on the Morton code (amr._key_move, amr._key_floor) equal the
+    Morton codes of the moved or rounded anchors, periodic wrap included
+    (anchors on the box's first and last cells of each level)."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    level = rng.integers(0, amr.GRID_BITS + 1, n)
+    h = amr._size(level)
+    cells = np.left_shift(np.int64(1), level.astype(np.int64))
+    anchor = rng.integers(0, cells[:, None], (n, 3)) * h[:, None]
+    anchor[:8] = 0
+    anchor[8:16] = ((cells[8:16] - 1) * h[8:16])[:, None]
+    key = amr._mkeys(anchor)
+    for d in range(3):
+        for s in (-1, 1):
+            moved = anchor.copy()
+            moved[:, d] = (moved[:, d] + s * h) % amr.ROOT
+            assert np.array_equal(amr._key_move(key, d, amr._dilated(h, d), s), amr._mkeys(moved))
+    big = np.minimum(2 * h, amr.ROOT)
+    assert np.array_equal(amr._key_floor(key, big), amr._mkeys(anchor & ~(big - 1)[:, None]))
+
+
 class _OracleSolver:
     """The calls amr_run.AdaptiveRun makes of a solver, answered by the
     oracle (indicator, CPU steps with the forest's mortars, state transfer)."""

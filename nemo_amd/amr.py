@@ -208,23 +208,44 @@ class Forest:
 
         The leaves tile the box in Morton order, so the leaf containing a
         lattice point is the last one whose anchor key is not above the
-        point's key."""
-        f = self
+        point's key.  A leaf needs refining when a face neighbour point of
+        a leaf two or more levels finer lies in it.  After the first round a
+        violation can only come from a leaf just made (refining a leaf only
+        makes the leaves around it satisfy the rule more easily) or persist
+        at a leaf whose neighbour was refined but is still too coarse, so
+        later rounds look from those leaves alone, and the Morton codes of
+        the other leaves are carried over."""
+        level, anchor = self.level, self.anchor
+        mk = _mkeys(anchor)
+        look = None  # the leaves to look from: all, then the new and the failed ones
         while True:
-            mk = _mkeys(f.anchor)
             keys = mk.astype(np.int64)
-            h = _size(f.level)
-            need = np.zeros(f.nelem, dtype=bool)
+            lv = level if look is None else level[look]
+            mq = mk if look is None else mk[look]
+            h = _size(lv)
+            need = np.zeros(len(level), dtype=bool)
+            failed = np.zeros(len(lv), dtype=bool)
             for d in range(3):
                 step = _dilated(h, d)
                 for s in (-1, 1):
-                    kp = _key_move(mk, d, step, s).astype(np.int64)
-                    c = np.searchsorted(keys, kp, side="right") - 1
-                    bad = f.level[c] < f.level - 1
+                    c = np.searchsorted(keys, _key_move(mq, d, step, s).astype(np.int64), side="right") - 1
+                    bad = level[c] < lv - 1
                     need[c[bad]] = True
+                    failed |= bad
             if not need.any():
-                return f
-            f = f.refine(need)
+                if look is None:
+                    return self
+                return Forest(level, anchor, self.minlevel, self.maxlevel, self.extent, self.origin)
+            again = np.zeros(len(level), dtype=bool)
+            again[np.arange(len(level)) if look is None else look] = failed
+            lch, ach = _children(level[need], anchor[need])
+            level = np.concatenate([level[~need], lch])
+            anchor = np.concatenate([anchor[~need], ach])
+            mk = np.concatenate([mk[~need], _mkeys(ach)])
+            again = np.concatenate([again[~need], np.ones(len(lch), dtype=bool)])
+            p = np.argsort(mk, kind="stable")
+            level, anchor, mk, again = level[p], anchor[p], mk[p], again[p]
+            look = np.nonzero(again)[0]
 
     # ---- geometry and connectivity ---------------------------------------
     @property

@@ -184,6 +184,50 @@ def test_face_neighbour_flags_by_geometry():
         assert np.array_equal(amr_run.face_neighbours_any(f, flag), want)
 
 
+def _touching(f):
+    """[n, n] leaves sharing a face (periodic), by brute force on the boxes."""
+    lo = f.anchor.astype(np.int64)
+    h = (amr.ROOT >> f.level).astype(np.int64)
+    n = f.nelem
+    touch = np.zeros((n, n), dtype=bool)
+    for d in range(3):
+        for s in (-1, 1):
+            plane_i = (lo[:, d] + (h if s > 0 else 0)) % amr.ROOT
+            plane_j = (lo[:, d] + (0 if s > 0 else h)) % amr.ROOT
+            ov = plane_i[:, None] == plane_j[None, :]
+            for t in range(3):
+                if t != d:
+                    ov &= (lo[:, None, t] < lo[None, :, t] + h[None, :]) & (lo[None, :, t] < lo[:, None, t] + h[:, None])
+            touch |= ov
+    np.fill_diagonal(touch, False)
+    return touch
+
+
+def test_balance_matches_brute_force_rounds():
+    """Forest.balance (rounds that look only from new leaves and from leaves
+    whose check failed) equals refining, round by round until none is left,
+    every leaf that shares a face with a leaf two or more levels finer,
+    found by brute force on the boxes; inputs refined several times without
+    balancing (jumps of up to four levels)."""
+    rng = np.random.default_rng(12)
+    for trial in range(12):
+        f = amr.Forest.uniform(1, minlevel=1, maxlevel=5)
+        for r in range(3):
+            flag = (rng.random(f.nelem) < 0.2) & (f.level < 5)
+            if r:
+                flag |= (f.level == f.level.max()) & (rng.random(f.nelem) < 0.5) & (f.level < 5)
+            f = f.refine(flag)
+        g = f
+        while True:
+            t = _touching(g)
+            need = (t & (g.level[None, :] > g.level[:, None] + 1)).any(axis=1)
+            if not need.any():
+                break
+            g = g.refine(need)
+        b = f.balance()
+        assert np.array_equal(b.level, g.level) and np.array_equal(b.anchor, g.anchor), trial
+
+
 def test_morton_code_moves():
     """Moves on the Morton code (amr._key_move, amr._key_floor) equal the
     Morton codes of the moved or rounded anchors, periodic wrap included

@@ -322,16 +322,34 @@ __device__ __forceinline__ void direction_pass(const StageArgs<M, Real>& a, long
         // low-storage/4-5/timedisc_probe_mod.f90:31-37, as probe_lsrk
         const int s = a.stage;
         const long long EM3 = a.nelem * (long long)M3;
+        double regs[5][M];
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const int n = bslot<M>(base, nidx<D, M>(i, ta, tb));
-            const long long gn = e * M3 + nidx<D, M>(i, ta, tb);
 #pragma unroll
             for (int h = 0; h < 5; ++h) {
                 const double rhs = s_acc[h][n] + c[i][h];
                 const double reg = (s == 1) ? rhs : a.rk.lsrk_a[s - 1] * xu->r0[h][i] + rhs;
-                if (valid) a.r0[h * EM3 + gn] = reg;
+                regs[h][i] = reg;
                 xu->unew[h][i] = S[i].u[h] + a.rk.lsrk_c[s - 1] * xu->dt * reg;
+            }
+        }
+        // the register of the line's nodes, four consecutive doubles at N = 3
+        // (32-byte aligned): two 16-byte stores per variable
+        if (valid) {
+            if constexpr (D == 0 && M == 4) {
+                const long long g0 = e * M3 + nidx<D, M>(0, ta, tb);
+#pragma unroll
+                for (int h = 0; h < 5; ++h) {
+                    double2* dst = reinterpret_cast<double2*>(a.r0 + h * EM3 + g0);
+                    dst[0] = make_double2(regs[h][0], regs[h][1]);
+                    dst[1] = make_double2(regs[h][2], regs[h][3]);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < M; ++i)
+#pragma unroll
+                    for (int h = 0; h < 5; ++h) a.r0[h * EM3 + e * M3 + nidx<D, M>(i, ta, tb)] = regs[h][i];
             }
         }
     } else {

@@ -745,8 +745,13 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
                         const long long eb = (e0 + lel) * M3;
 #pragma unroll
                         for (int h = 0; h < V; ++h)
-#pragma unroll
-                            for (int q = 0; q < QS; ++q) a.u_out[h * EM3 + eb + nidx<0, M>(q, lta, ltb)] = xu.unew[h][q];
+                            if constexpr (M == 4) {  // 16-byte stores of consecutive node pairs
+                                for (int q = 0; q < QS; q += 2)
+                                    *reinterpret_cast<double2*>(a.u_out + h * EM3 + eb + nidx<0, M>(q, lta, ltb)) =
+                                        make_double2(xu.unew[h][q], xu.unew[h][q + 1]);
+                            } else {
+                                for (int q = 0; q < QS; ++q) a.u_out[h * EM3 + eb + nidx<0, M>(q, lta, ltb)] = xu.unew[h][q];
+                            }
                     }
                     return true;
                 }

@@ -96,6 +96,7 @@ struct StageArgs {
     float* __restrict__ feat_out;    // non-null: write features [nelem][4] (training data)
     unsigned long long* __restrict__ mlp_stats;  // judged, overridden, fallback
     int alpha_refresh;               // 1: compute the factor (first stage of a step)
+    int alpha_only;                  // 1 (Euler): write the factor to alpha_out and stop there
     uint32_t step_index;             // sampling key of the judge
     uint32_t sample_thresh;          // judged fraction * 2^32
     double judge_tol;                // PP takes over where it exceeds the prediction by this

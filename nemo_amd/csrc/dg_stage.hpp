@@ -680,6 +680,11 @@ __device__ __forceinline__ void stage_block(const StageArgs<M, Real>& a, const l
         alpha = indicator_pp<EQ, M, BE, Real, TU>(a, e0, s_u, s_acc, s_p64);
     }
 
+    // indicator only (nemo_gpu_indicator, Euler): the factor has gone to
+    // alpha_out
+    if constexpr (EQ == EQ_EULER)
+        if (a.alpha_only) return;
+
     // ---- three direction passes ----------------------------------------
     if constexpr (EQ == EQ_MHD) {
         direction_pass_mhd<0, M, BE, VOL, SURF, Real>(a, mg, e0, s_u, s_acc, alpha);

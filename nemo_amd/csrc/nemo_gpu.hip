@@ -606,7 +606,10 @@ int nemo_gpu_indicator(void* handle, double* alpha) {
         for (long long e = 0; e < s->nelem; ++e) alpha[e] = 0.0;
         return NEMO_OK;
     }
-    if (int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, s->alpha)) return rc;
+    // the Euler stage kernel stops after the indicator (no operator, no RHS)
+    LaunchOpts o;
+    o.alpha_only = s->cfg.equations == NEMO_EQ_EULER;
+    if (int rc = launch_stage(*s, 1, 0.0, nullptr, s->rhs, s->alpha, nullptr, o)) return rc;
     HIP_TRY(hipMemcpyAsync(alpha, s->alpha, size_t(s->nelem) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return NEMO_OK;

@@ -75,6 +75,7 @@ struct Solver {
 struct LaunchOpts {
     int refresh = 1;            // learned indicator: compute the factor of the step
     float* feat_out = nullptr;  // training data: features of every element
+    int alpha_only = 0;         // indicator only (Euler): the factor to alpha_out, no operator
 };
 
 // Stage launch of one polynomial degree (M = n_poly + 1 nodes per direction),

@@ -20,6 +20,7 @@ nemo::StageArgs<M, Real> make_args(Solver& s, int stage, double dt, const double
     a.feat_out = o.feat_out;
     a.mlp_stats = s.mlp_stats;
     a.alpha_refresh = o.refresh;
+    a.alpha_only = o.alpha_only;
     a.step_index = s.step_counter;
     a.sample_thresh = s.sample_fraction >= 1.0 ? 0xFFFFFFFFu : uint32_t(s.sample_fraction * 4294967296.0);
     a.judge_tol = s.judge_tol;
@@ -124,8 +125,8 @@ int launch_stage_mr(Solver& s, int stage, double dt, const double* dt_dev, doubl
                     double* cfl_part, const LaunchOpts& o) {
     const auto a = make_args<M, Real>(s, stage, dt, dt_dev, rhs_out, alpha_out, cfl_part, o);
     const int vol = s.cfg.volume, surf = s.cfg.surface;
-    if (s.nmortar > 0) launch_mortar<M, Real>(s);
-    if (s.nbound > 0) launch_boundary<M>(s);
+    if (s.nmortar > 0 && !o.alpha_only) launch_mortar<M, Real>(s);
+    if (s.nbound > 0 && !o.alpha_only) launch_boundary<M>(s);
     if constexpr (M == 4) {
         // learned indicator, and the training-data pass of any Euler scheme
         if (s.cfg.equations == NEMO_EQ_EULER && (o.feat_out || s.cfg.indicator == NEMO_IND_MLP)) {

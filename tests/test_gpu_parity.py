@@ -196,50 +196,64 @@ def test_partial_last_block(N):
     assert _rel(u, uref) <= RUN_TOL
 
 
-def _near1_edge_state(m, N, BE=4, F=1.085):
-    """Blocks of BE elements (N = 3: four consecutive Morton elements, a
-    2x2x1 patch) whose density and pressure vary as f^g and f^-g around the
-    block's first node, g = sin(2 pi u) in the block's x coordinate u, with
-    f set per block so that the extreme node sits 0.1 % inside or 0.1 %
-    outside the stage kernel's near-1 bound (every node within a factor F of
-    the first node's density and pressure): both variants of the direction
-    passes run, the near-1 one at beta ratios up to F^4.  Returns the state
-    and the host's restatement of the block test."""
+# elements per workgroup of the stage kernel by degree (tiles of 7 x 9,
+# 4 x 16 and 2 x 25 lanes)
+TILE_BE = {2: 7, 3: 4, 4: 2}
+
+
+def _near1_edge_state(m, N, F=1.085):
+    """Blocks of the stage kernel's tile (N = 3: four consecutive Morton
+    elements, a 2x2x1 patch; N = 2: seven; N = 4: two) whose density and
+    pressure vary as f^g and f^-g around the block's first node, g = sin(2 pi
+    u) in the block's x coordinate u, with f set per block so that the
+    extreme node sits 0.1 % inside or 0.1 % outside the stage kernel's near-1
+    bound (every node within a factor F of the first node's density and
+    pressure): both variants of the direction passes run, the near-1 one at
+    beta ratios up to F^4.  Returns the state and the host's restatement of
+    the block test (a last partial block's padding slots hold element 0's
+    first node, density and pressure 1, as its own first node)."""
     from nemo_amd import operators
+    BE = TILE_BE[N]
     xi, _ = operators.lgl(N)
     X = m.node_coords(xi)                                  # [nelem, 3, M^3]
-    nb = m.nelem // BE
-    Xb = X.reshape(nb, BE, 3, -1)
-    x0 = Xb[:, 0, 0, 0][:, None, None]
-    u = (Xb[:, :, 0, :] - x0) / (2 * m.delta[0, 0])
-    g = np.sin(2 * np.pi * u)                              # 0 at the block's first node
-    gmax = np.maximum(g.max(axis=(1, 2)), -g.min(axis=(1, 2)))
-    inside = (np.arange(nb) % 2) == 0
-    lnf = np.log(F) / gmax * np.where(inside, 0.999, 1.001)
-    rho = np.exp(lnf[:, None, None] * g).reshape(m.nelem, -1)
-    p = np.exp(-lnf[:, None, None] * g).reshape(m.nelem, -1)
+    nb = -(-m.nelem // BE)
+    rho = np.empty((m.nelem, X.shape[2]))
+    p = np.empty_like(rho)
+    for b in range(nb):
+        els = np.arange(b * BE, min((b + 1) * BE, m.nelem))
+        x = X[els, 0, :]
+        u = (x - X[els[0], 0, 0]) / max(np.ptp(x), 1e-300)
+        g = np.sin(2 * np.pi * u)                          # 0 at the block's first node
+        gmax = max(np.abs(g).max(), 1e-3)
+        lnf = np.log(F) / gmax * (0.999 if b % 2 == 0 else 1.001)
+        rho[els] = np.exp(lnf * g)
+        p[els] = np.exp(-lnf * g)
     vx, vy, vz = 0.3 + 0 * rho, -0.2 + 0 * rho, 0.1 + 0 * rho
     U = setups.prim2cons(rho, vx, vy, vz, p, 1.4)           # [nelem, 5, M^3]
     # the kernel's test, restated: pressures from the conserved variables
-    r = U[:, 0].reshape(nb, -1)
-    pp = (0.4 * (U[:, 4] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / U[:, 0])).reshape(nb, -1)
-    rr, pr = r[:, :1], pp[:, :1]
-    ok = ((r <= F * rr) & (rr <= F * r) & (pp <= F * pr) & (pr <= F * pp)).all(axis=1)
+    pp = 0.4 * (U[:, 4] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / U[:, 0])
+    ok = np.empty(nb, dtype=bool)
+    for b in range(nb):
+        els = np.arange(b * BE, min((b + 1) * BE, m.nelem))
+        r, q = U[els, 0].ravel(), pp[els].ravel()
+        rr, pr = r[0], q[0]
+        ok[b] = ((r <= F * rr) & (rr <= F * r) & (q <= F * pr) & (pr <= F * q)).all()
     return np.ascontiguousarray(U), ok
 
 
-def test_near1_log_means_at_their_bound():
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_near1_log_means_at_their_bound(N):
     """The FP64 two-point kernel's two variants of the log-means (with and
     without the log's range reduction, chosen per block) against the oracle
     on blocks built to sit just inside and just outside the choice's bound:
-    RHS to 1e-12 and 20 LSRK steps (the near-1 variant fuses the update into
-    its x pass, the general one leaves it to the separate update) to 1e-10."""
-    N = 3
+    RHS to 1e-12 and 20 LSRK steps to 1e-10 (at N = 2, 3 the near-1 variant
+    fuses the update into its x pass and the general one leaves it to the
+    separate update; at N = 4 both leave it to the separate update)."""
     m = meshmod.uniform_periodic(8)
     u0, ok = _near1_edge_state(m, N)
     assert ok.any() and not ok.all()
     g, o = _cfgs(N, _lib.VOL_EC, _lib.SURF_HLLC, _lib.IND_PP, _lib.TD_LSRK54, repair=1)
-    print(f"near-1 bound: {int(ok.sum())} of {ok.size} blocks inside", flush=True)
+    print(f"near-1 bound, N = {N}: {int(ok.sum())} of {ok.size} blocks inside", flush=True)
     with solver.Solver(g) as s:
         s.upload_mesh(m)
         s.set_state(u0)

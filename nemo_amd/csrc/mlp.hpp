@@ -11,8 +11,10 @@
 // Parameter vector (FP32, canonical layout, NPARAMS = 8705):
 //   W1[64][4], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4
 // with W[j][k] the weight from input k to unit j.  The inference layout
-// used by the stage kernel stores W2 and W3 transposed (k-major) so that a
-// wavefront reading W[.][k] for its 64 units reads 256 contiguous bytes.
+// used by the stage kernel stores W2 and W3 in blocks of four inputs,
+// W[j][4b..4b+3] at 4 (64 b + j): lane j reads its unit's four weights as one
+// 16-byte load and a wavefront reads 1 KB contiguous per block (16 loads per
+// layer instead of 64).
 //
 // Kernels map one hidden unit to one lane of a 64-lane wavefront; an
 // activation of another unit is read with v_readlane (uniform index).
@@ -78,10 +80,15 @@ __device__ __forceinline__ void mlp_forward_wave(const float* __restrict__ P, co
         const float b = P[MLP_B2 + j];
 #pragma unroll
         for (int s = 0; s < NB; ++s) z[s] = b;
-        for (int k = 0; k < MLP_H; ++k) {
-            const float w = P[MLP_W2 + k * MLP_H + j];  // transposed: W2[j][k]
+        for (int kb = 0; kb < MLP_H / 4; ++kb) {
+            const float4 w = *reinterpret_cast<const float4*>(P + MLP_W2 + kb * 4 * MLP_H + j * 4);
 #pragma unroll
-            for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a1[s], k);
+            for (int s = 0; s < NB; ++s) {
+                z[s] += w.x * lane_bcast(a1[s], 4 * kb);
+                z[s] += w.y * lane_bcast(a1[s], 4 * kb + 1);
+                z[s] += w.z * lane_bcast(a1[s], 4 * kb + 2);
+                z[s] += w.w * lane_bcast(a1[s], 4 * kb + 3);
+            }
         }
 #pragma unroll
         for (int s = 0; s < NB; ++s) a2[s] = gelu(z[s]);
@@ -91,10 +98,15 @@ __device__ __forceinline__ void mlp_forward_wave(const float* __restrict__ P, co
         const float b = P[MLP_B3 + j];
 #pragma unroll
         for (int s = 0; s < NB; ++s) z[s] = b;
-        for (int k = 0; k < MLP_H; ++k) {
-            const float w = P[MLP_W3 + k * MLP_H + j];
+        for (int kb = 0; kb < MLP_H / 4; ++kb) {
+            const float4 w = *reinterpret_cast<const float4*>(P + MLP_W3 + kb * 4 * MLP_H + j * 4);
 #pragma unroll
-            for (int s = 0; s < NB; ++s) z[s] += w * lane_bcast(a2[s], k);
+            for (int s = 0; s < NB; ++s) {
+                z[s] += w.x * lane_bcast(a2[s], 4 * kb);
+                z[s] += w.y * lane_bcast(a2[s], 4 * kb + 1);
+                z[s] += w.z * lane_bcast(a2[s], 4 * kb + 2);
+                z[s] += w.w * lane_bcast(a2[s], 4 * kb + 3);
+            }
         }
 #pragma unroll
         for (int s = 0; s < NB; ++s) a3[s] = gelu(z[s]);
@@ -240,16 +252,16 @@ static __global__ void mlp_reduce_kernel(const float* __restrict__ part, int npa
     }
 }
 
-// Canonical -> inference layout (W2 and W3 transposed).
+// Canonical -> inference layout (W2 and W3 in blocks of four inputs).
 static __global__ void mlp_inference_layout_kernel(const float* __restrict__ P, float* __restrict__ Q) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= MLP_NP) return;
     if (p >= MLP_W2 && p < MLP_B2) {
         const int r = (p - MLP_W2) / MLP_H, c = (p - MLP_W2) % MLP_H;
-        Q[MLP_W2 + c * MLP_H + r] = P[p];
+        Q[MLP_W2 + (c >> 2) * 4 * MLP_H + r * 4 + (c & 3)] = P[p];
     } else if (p >= MLP_W3 && p < MLP_B3) {
         const int r = (p - MLP_W3) / MLP_H, c = (p - MLP_W3) % MLP_H;
-        Q[MLP_W3 + c * MLP_H + r] = P[p];
+        Q[MLP_W3 + (c >> 2) * 4 * MLP_H + r * 4 + (c & 3)] = P[p];
     } else {
         Q[p] = P[p];
     }

@@ -5,7 +5,7 @@
 //
 // The reference has no learned indicator (its blending traces are in
 // setups/mhd/3d/_common_/setup_checkpoint_mod.f90:395-450); this is the
-// build's own network, trained on (features, alpha) pairs that the
+// library's own network, trained on (features, alpha) pairs that the
 // Persson-Peraire indicator produces during the method's own runs.
 //
 // Parameter vector (FP32, canonical layout, NPARAMS = 8705):

@@ -6,7 +6,7 @@ include/nemo_gpu.h) predicts the DG/FV blending factor from the log modal
 energy fractions of rho and p.  Training data are (features, alpha_PP) pairs
 that nemo_gpu_indicator_features dumps during plain-indicator runs of Sedov
 blasts with seeded centre jitter and of the Kelvin-Helmholtz shear layer;
-training runs on the GPU (nemo_gpu_mlp_train: the build's own forward,
+training runs on the GPU (nemo_gpu_mlp_train: the library's own forward,
 backward and Adam kernels).
 
 Run as a script to regenerate the committed parameters:
